@@ -1,0 +1,262 @@
+"""MI355X (gfx950) bitmask stream compaction -- Python binding of libcsx.so.
+
+The compute path is the C ABI in ``include/csx.h``, implemented by
+hand-written HIP kernels in ``gpu_compressstore2_amd/csrc``.  This module
+only marshals torch device tensors (device memory and streams are torch's)
+into those calls.  It never computes anything itself: if ``libcsx.so`` is
+missing, every call raises.
+
+Functions mirror the reference's phase launchers
+(RememberOfLife/gpu_compressstore2, src/kernels/kernel_3pass.cuh):
+  popc        launch_3pass_popc_none         kernel_3pass.cuh:40-57
+  scan        launch_3pass_pss_gmem + pss2   kernel_3pass.cuh:107-214
+  scan_tree   launch_3pass_pss_gmem          kernel_3pass.cuh:107-153
+  tree_offsets launch_3pass_pss2_gmem        kernel_3pass.cuh:188-214
+  writeout    launch_3pass_proc_true/none    kernel_3pass.cuh:536-693
+  compact     benchN compositions            src/benchmarks.cuh:153-440
+"""
+import ctypes
+import os
+
+import torch
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libcsx.so")
+
+MSB_FIRST = 0
+LSB_FIRST = 1
+
+WRITEOUT_STAGED = 0
+WRITEOUT_DIRECT = 1
+WRITEOUT_THREAD_PER_CHUNK = 2
+
+PIPELINE_FUSED = 0
+PIPELINE_THREE_PHASE = 1
+
+MAX_ELEMENTS = 1 << 40
+
+STATUS = {
+    0: "CSX_OK",
+    1: "CSX_ERR_INVALID",
+    2: "CSX_ERR_WORKSPACE",
+    3: "CSX_ERR_HIP",
+    4: "CSX_ERR_UNSUPPORTED",
+    5: "CSX_ERR_DEVICE",
+}
+
+
+class CsxError(RuntimeError):
+    """A csx_* call returned a status other than CSX_OK."""
+
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+_u32 = ctypes.c_uint32
+_int = ctypes.c_int
+_sz = ctypes.c_size_t
+
+# name -> (restype, argtypes); the exported surface of include/csx.h
+SIGNATURES = {
+    "csx_status_string": (ctypes.c_char_p, [_int]),
+    "csx_last_error": (ctypes.c_char_p, []),
+    "csx_version": (_int, []),
+    "csx_chunk_count": (_u64, [_u64, _u32]),
+    "csx_popc": (_int, [_vp, _vp, _u64, _u32, _int, _vp]),
+    "csx_scan_workspace_size": (_int, [_u64, ctypes.POINTER(_sz)]),
+    "csx_scan": (_int, [_vp, _vp, _u64, _vp, _vp, _sz, _vp]),
+    "csx_scan_tree": (_int, [_vp, _u64, _vp, _vp, _sz, _vp]),
+    "csx_tree_offsets": (_int, [_vp, _vp, _u64, _vp]),
+    "csx_writeout": (_int, [_vp, _vp, _vp, _vp, _vp, _u64, _u32, _u32, _int, _int, _vp]),
+    "csx_writeout_w1": (_int, [_vp, _vp, _vp, _vp, _vp, _u64, _u32, _int, _int, _vp]),
+    "csx_writeout_w2": (_int, [_vp, _vp, _vp, _vp, _vp, _u64, _u32, _int, _int, _vp]),
+    "csx_writeout_w4": (_int, [_vp, _vp, _vp, _vp, _vp, _u64, _u32, _int, _int, _vp]),
+    "csx_writeout_w8": (_int, [_vp, _vp, _vp, _vp, _vp, _u64, _u32, _int, _int, _vp]),
+    "csx_workspace_size": (_int, [_u64, _u32, _u32, ctypes.POINTER(_sz)]),
+    "csx_compact": (_int, [_vp, _vp, _vp, _u64, _u32, _u32, _int, _int, _vp, _vp, _vp, _sz, _vp]),
+    "csx_compact_w1": (_int, [_vp, _vp, _vp, _u64, _u32, _int, _int, _vp, _vp, _vp, _sz, _vp]),
+    "csx_compact_w2": (_int, [_vp, _vp, _vp, _u64, _u32, _int, _int, _vp, _vp, _vp, _sz, _vp]),
+    "csx_compact_w4": (_int, [_vp, _vp, _vp, _u64, _u32, _int, _int, _vp, _vp, _vp, _sz, _vp]),
+    "csx_compact_w8": (_int, [_vp, _vp, _vp, _u64, _u32, _int, _int, _vp, _vp, _vp, _sz, _vp]),
+    "csx_workspace_error": (_int, [_vp, ctypes.POINTER(_u32), _vp]),
+    "csx_validate": (_int, [_vp, _vp, _u64, _u32, _vp, _vp]),
+    "csx_profile_begin": (_int, []),
+    "csx_profile_end": (_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
+}
+
+_LIB = None
+
+
+def lib():
+    """The loaded libcsx.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m gpu_compressstore2_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            if hasattr(L, name):
+                f = getattr(L, name)
+                f.restype = res
+                f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(status):
+    if status != 0:
+        msg = lib().csx_last_error().decode()
+        raise CsxError(status, msg)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _require_device(*ts):
+    for t in ts:
+        if t is not None and not t.is_cuda:
+            raise ValueError("csx operates on device tensors")
+        if t is not None and not t.is_contiguous():
+            raise ValueError("csx needs contiguous tensors")
+
+
+def chunk_count(n, chunk_len):
+    return (n + chunk_len - 1) // chunk_len
+
+
+def workspace_size(n, elem_size, chunk_len=1024):
+    out = _sz(0)
+    _check(lib().csx_workspace_size(n, elem_size, chunk_len, ctypes.byref(out)))
+    return out.value
+
+
+def scan_workspace_size(chunks):
+    out = _sz(0)
+    _check(lib().csx_scan_workspace_size(chunks, ctypes.byref(out)))
+    return out.value
+
+
+def alloc_workspace(nbytes, device=None):
+    return torch.empty(max(int(nbytes), 16), dtype=torch.uint8, device=device or "cuda")
+
+
+def popc(mask, n, chunk_len=1024, bit_order=MSB_FIRST, counts=None, stream=None):
+    """Per-chunk selected counts (u32) of a packed mask of n elements."""
+    _require_device(mask, counts)
+    if counts is None:
+        counts = torch.empty(max(chunk_count(n, chunk_len), 1), dtype=torch.int32, device=mask.device)
+    _check(lib().csx_popc(_ptr(mask), _ptr(counts), n, chunk_len, bit_order, _stream(stream)))
+    return counts[: chunk_count(n, chunk_len)]
+
+
+def scan(counts, offsets=None, total=None, workspace=None, stream=None):
+    """Exclusive prefix sum of u32 chunk counts -> (u64 offsets, u64 total tensor)."""
+    _require_device(counts)
+    m = counts.numel()
+    dev = counts.device
+    if offsets is None:
+        offsets = torch.empty(max(m, 1), dtype=torch.int64, device=dev)
+    if total is None:
+        total = torch.empty(1, dtype=torch.int64, device=dev)
+    if workspace is None:
+        workspace = alloc_workspace(scan_workspace_size(m), dev)
+    _check(lib().csx_scan(_ptr(counts), _ptr(offsets), m, _ptr(total), _ptr(workspace), workspace.numel(), _stream(stream)))
+    return offsets[:m], total
+
+
+def scan_tree(pss, total=None, workspace=None, stream=None):
+    """In place: u32 counts -> the reference's up-swept tree; returns (pss, total)."""
+    _require_device(pss)
+    m = pss.numel()
+    if total is None:
+        total = torch.empty(1, dtype=torch.int64, device=pss.device)
+    if workspace is None:
+        workspace = alloc_workspace(scan_workspace_size(m), pss.device)
+    _check(lib().csx_scan_tree(_ptr(pss), m, _ptr(total), _ptr(workspace), workspace.numel(), _stream(stream)))
+    return pss, total
+
+
+def tree_offsets(tree, out=None, stream=None):
+    """u32 exclusive offsets from an up-swept tree (the reference's tree walk)."""
+    _require_device(tree)
+    m = tree.numel()
+    if out is None:
+        out = torch.empty(max(m, 1), dtype=torch.int32, device=tree.device)
+    _check(lib().csx_tree_offsets(_ptr(tree), _ptr(out), m, _stream(stream)))
+    return out[:m]
+
+
+def writeout(inp, mask, offsets, n, chunk_len=1024, bit_order=MSB_FIRST, variant=WRITEOUT_STAGED, counts=None, out=None, stream=None):
+    """Phase 3: selected elements of inp to out at the chunk offsets."""
+    _require_device(inp, mask, offsets, counts, out)
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=inp.dtype, device=inp.device)
+    _check(
+        lib().csx_writeout(
+            _ptr(inp), _ptr(mask), _ptr(offsets), _ptr(counts), _ptr(out), n, inp.element_size(), chunk_len, bit_order, variant, _stream(stream)
+        )
+    )
+    return out
+
+
+def compact(inp, mask, n=None, chunk_len=1024, bit_order=MSB_FIRST, pipeline=PIPELINE_FUSED, out=None, count=None, workspace=None,
+            stream=None, host_count=True):
+    """Full compaction.  Returns (out, k) with k an int (host_count) or a device tensor."""
+    _require_device(inp, mask, out, count)
+    n = inp.numel() if n is None else n
+    dev = inp.device
+    w = inp.element_size()
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=inp.dtype, device=dev)
+    if count is None:
+        count = torch.empty(1, dtype=torch.int64, device=dev)
+    if workspace is None:
+        workspace = alloc_workspace(workspace_size(n, w, chunk_len), dev)
+    h = _u64(0)
+    _check(
+        lib().csx_compact(
+            _ptr(inp), _ptr(mask), _ptr(out), n, w, chunk_len, bit_order, pipeline, _ptr(count),
+            ctypes.byref(h) if host_count else None, _ptr(workspace), workspace.numel(), _stream(stream),
+        )
+    )
+    return out, (int(h.value) if host_count else count)
+
+
+def validate(expected, actual, length, failures=None, stream=None):
+    """Device count of positions < length where expected and actual differ."""
+    _require_device(expected, actual)
+    if failures is None:
+        failures = torch.zeros(1, dtype=torch.int64, device=expected.device)
+    _check(lib().csx_validate(_ptr(expected), _ptr(actual), length, expected.element_size(), _ptr(failures), _stream(stream)))
+    return failures
+
+
+def profile_begin():
+    """Start timing the dominant kernel of every compact/writeout call on this thread."""
+    _check(lib().csx_profile_begin())
+
+
+def profile_end():
+    """(summed milliseconds, number of timed launches) since profile_begin()."""
+    ms = ctypes.c_double(0.0)
+    cnt = _u64(0)
+    _check(lib().csx_profile_end(ctypes.byref(ms), ctypes.byref(cnt)))
+    return ms.value, cnt.value
+
+
+def workspace_error(workspace, stream=None):
+    e = _u32(0)
+    _check(lib().csx_workspace_error(_ptr(workspace), ctypes.byref(e), _stream(stream)))
+    return e.value

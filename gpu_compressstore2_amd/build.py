@@ -1,0 +1,86 @@
+"""Build the gfx950 compaction library in-tree.
+
+    python -m gpu_compressstore2_amd.build      # -> gpu_compressstore2_amd/libcsx.so
+
+Every HIP translation unit is compiled with hipcc for gfx950 only; objects go
+to gpu_compressstore2_amd/_build/, the shared library next to this file.
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "libcsx.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = [
+    "-std=c++17",
+    "-O3",
+    "-fPIC",
+    "-Wall",
+    "-Wno-unused-function",
+    f"-I{INCLUDE}",
+    f"-I{CSRC}",
+]
+DEVICE = [
+    f"--offload-arch={ARCH}",
+    # packed FP32 instructions stay off in every kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics",
+]
+
+
+def _units():
+    """(source, object, extra flags) for every translation unit."""
+    units = [
+        ("csx_core.hip", "csx_core.o", DEVICE),
+        ("csx_api.cpp", "csx_api.o", []),
+    ]
+    for w in (1, 2, 4, 8):
+        units.append(("csx_tile.hip", f"csx_tile_w{w}.o", DEVICE + [f"-DCSX_TILE_W={w}"]))
+    for extra in ("csx_pattern.hip", "csx_gen.hip", "csx_batch.hip"):
+        if os.path.exists(os.path.join(CSRC, extra)):
+            units.append((extra, extra.replace(".hip", ".o"), DEVICE))
+    return units
+
+
+def _compile(unit):
+    src, obj, flags = unit
+    src_path = os.path.join(CSRC, src)
+    obj_path = os.path.join(BUILD, obj)
+    lang = ["-x", "hip"] if src.endswith(".hip") else []
+    cmd = [HIPCC, *lang, *COMMON, *flags, "-c", src_path, "-o", obj_path]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj_path
+
+
+def build(jobs=None):
+    """Compile every unit (always; no up-to-date check) and link libcsx.so."""
+    if not os.path.exists(HIPCC):
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    os.makedirs(BUILD, exist_ok=True)
+    units = _units()
+    jobs = jobs or min(len(units), max(1, (os.cpu_count() or 4)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, units))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    shutil.move(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build())
+    sys.exit(0)

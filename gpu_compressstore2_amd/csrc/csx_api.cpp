@@ -1,0 +1,466 @@
+// csx_api.cpp -- the C ABI (include/csx.h): argument checks, workspace
+// planning and dispatch to the gfx950 kernels.  No call here allocates
+// device memory; scratch lives in the caller's workspace.
+#include "csx.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "csx_internal.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+csx_status fail(csx_status s, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return s;
+}
+
+csx_status ok()
+{
+    g_last_error.clear();
+    return CSX_OK;
+}
+
+csx_status hip_fail(hipError_t e, const char* where)
+{
+    return fail(CSX_ERR_HIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+#define CSX_HIP(call, where)                                 \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return hip_fail(e_, (where));  \
+    } while (0)
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+bool pow2(uint64_t v) { return v && !(v & (v - 1)); }
+
+uint32_t log2u(uint64_t v)
+{
+    uint32_t r = 0;
+    while ((1ull << r) < v) r++;
+    return r;
+}
+
+csx_status check_chunk(uint32_t chunk_len)
+{
+    if (!pow2(chunk_len) || chunk_len < 32)
+        return fail(CSX_ERR_INVALID, "chunk length %u must be a power of two >= 32", chunk_len);
+    return CSX_OK;
+}
+
+csx_status check_width(uint32_t w)
+{
+    if (w != 1 && w != 2 && w != 4 && w != 8) return fail(CSX_ERR_INVALID, "element size %u must be 1, 2, 4 or 8", w);
+    return CSX_OK;
+}
+
+csx_status check_order(int order)
+{
+    if (order != CSX_MSB_FIRST && order != CSX_LSB_FIRST) return fail(CSX_ERR_INVALID, "bit order %d is neither MSB- nor LSB-first", order);
+    return CSX_OK;
+}
+
+csx_status check_n(uint64_t n)
+{
+    if (n > CSX_MAX_ELEMENTS) return fail(CSX_ERR_INVALID, "element count %llu exceeds the maximum %llu", (unsigned long long)n,
+                                          (unsigned long long)CSX_MAX_ELEMENTS);
+    return CSX_OK;
+}
+
+uint64_t tile_elems(uint32_t w)
+{
+    switch (w) {
+        case 1: return csx::kTileElems<1>;
+        case 2: return csx::kTileElems<2>;
+        case 4: return csx::kTileElems<4>;
+        default: return csx::kTileElems<8>;
+    }
+}
+
+uint64_t chunk_count(uint64_t n, uint32_t chunk_len) { return (n + chunk_len - 1) / chunk_len; }
+
+// Scan workspace: header | status[scan tiles] | u64[chunks] (tree scratch)
+struct ScanPlan {
+    uint64_t tiles;
+    size_t status_off, tmp_off, total;
+};
+
+ScanPlan plan_scan(uint64_t chunks)
+{
+    ScanPlan p;
+    p.tiles = (chunks + csx::kScanTileElems - 1) / csx::kScanTileElems;
+    p.status_off = sizeof(csx::WsHeader);
+    p.tmp_off = align_up(p.status_off + p.tiles * 8);
+    p.total = align_up(p.tmp_off + chunks * 8);
+    return p;
+}
+
+// Compaction workspace: header | status[max(fused tiles, scan tiles)] |
+// u32 counts[chunks] | u64 offsets[chunks].  The two pipelines share the
+// header and the status words.
+struct CompactPlan {
+    uint64_t fused_tiles, scan_tiles, chunks;
+    size_t status_off, counts_off, offsets_off, total;
+};
+
+CompactPlan plan_compact(uint64_t n, uint32_t w, uint32_t chunk_len)
+{
+    CompactPlan p;
+    const uint64_t te = tile_elems(w);
+    p.fused_tiles = (n + te - 1) / te;
+    p.chunks = chunk_count(n, chunk_len);
+    p.scan_tiles = (p.chunks + csx::kScanTileElems - 1) / csx::kScanTileElems;
+    const uint64_t st = p.fused_tiles > p.scan_tiles ? p.fused_tiles : p.scan_tiles;
+    p.status_off = sizeof(csx::WsHeader);
+    p.counts_off = align_up(p.status_off + st * 8);
+    p.offsets_off = align_up(p.counts_off + p.chunks * 4);
+    p.total = align_up(p.offsets_off + p.chunks * 8);
+    return p;
+}
+
+csx_status check_ws(const void* ws, size_t have, size_t need)
+{
+    if (!ws) return fail(CSX_ERR_WORKSPACE, "workspace pointer is null (%zu bytes needed)", need);
+    if (((uintptr_t)ws & 15) != 0) return fail(CSX_ERR_WORKSPACE, "workspace must be 16-byte aligned");
+    if (have < need) return fail(CSX_ERR_WORKSPACE, "workspace of %zu bytes is smaller than the %zu bytes needed", have, need);
+    return CSX_OK;
+}
+
+hipError_t dispatch_tile(uint32_t w, int mode, bool staged, int order, bool aligned, const csx::TileArgs& a, hipStream_t s)
+{
+    switch (w) {
+        case 1: return csx::launch_tile<1>(mode, staged, order, aligned, a, s);
+        case 2: return csx::launch_tile<2>(mode, staged, order, aligned, a, s);
+        case 4: return csx::launch_tile<4>(mode, staged, order, aligned, a, s);
+        default: return csx::launch_tile<8>(mode, staged, order, aligned, a, s);
+    }
+}
+
+hipError_t dispatch_tpc(uint32_t w, int order, const uint8_t* in, const uint8_t* mask, const uint64_t* offsets, const uint32_t* counts,
+                        uint8_t* out, uint64_t n, uint32_t log2c, hipStream_t s)
+{
+    switch (w) {
+        case 1: return csx::launch_thread_per_chunk<1>(order, in, mask, offsets, counts, out, n, log2c, s);
+        case 2: return csx::launch_thread_per_chunk<2>(order, in, mask, offsets, counts, out, n, log2c, s);
+        case 4: return csx::launch_thread_per_chunk<4>(order, in, mask, offsets, counts, out, n, log2c, s);
+        default: return csx::launch_thread_per_chunk<8>(order, in, mask, offsets, counts, out, n, log2c, s);
+    }
+}
+
+bool launch_fits(uint64_t tiles) { return tiles * 256ull < (1ull << 32); }
+
+// ---- dominant-kernel timing (csx_profile_begin / csx_profile_end) ----
+struct Profiler {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+};
+thread_local Profiler g_prof;
+constexpr size_t kMaxProfiled = 1u << 16;
+
+// Launch `fn` (returning hipError_t) between a recorded event pair when profiling.
+template <typename F> hipError_t timed(hipStream_t s, F&& fn)
+{
+    if (!g_prof.on || g_prof.events.size() >= kMaxProfiled) return fn();
+    hipEvent_t a, b;
+    hipError_t e = hipEventCreate(&a);
+    if (e != hipSuccess) return e;
+    e = hipEventCreate(&b);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return e;
+    }
+    g_prof.events.emplace_back(a, b);
+    if ((e = hipEventRecord(a, s)) != hipSuccess) return e;
+    if ((e = fn()) != hipSuccess) return e;
+    return hipEventRecord(b, s);
+}
+
+} // namespace
+
+extern "C" {
+
+const char* csx_status_string(csx_status s)
+{
+    switch (s) {
+        case CSX_OK: return "ok";
+        case CSX_ERR_INVALID: return "invalid argument";
+        case CSX_ERR_WORKSPACE: return "workspace missing or too small";
+        case CSX_ERR_HIP: return "HIP runtime error";
+        case CSX_ERR_UNSUPPORTED: return "unsupported request";
+        case CSX_ERR_DEVICE: return "device-side failure";
+    }
+    return "unknown status";
+}
+
+const char* csx_last_error(void) { return g_last_error.c_str(); }
+
+int csx_version(void) { return 100; }
+
+uint64_t csx_chunk_count(uint64_t n, uint32_t chunk_len) { return chunk_len ? chunk_count(n, chunk_len) : 0; }
+
+csx_status csx_popc(const uint8_t* d_mask, uint32_t* d_counts, uint64_t n, uint32_t chunk_len, int bit_order, hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_chunk(chunk_len)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
+    if (n == 0) return ok();
+    if (!d_mask || !d_counts) return fail(CSX_ERR_INVALID, "null mask or counts pointer");
+    CSX_HIP(csx::launch_popc(d_mask, d_counts, n, log2u(chunk_len), bit_order, stream), "csx_popc launch");
+    return ok();
+}
+
+csx_status csx_scan_workspace_size(uint64_t chunks, size_t* bytes)
+{
+    if (!bytes) return fail(CSX_ERR_INVALID, "null size pointer");
+    *bytes = plan_scan(chunks).total;
+    return ok();
+}
+
+csx_status csx_scan(const uint32_t* d_counts, uint64_t* d_offsets, uint64_t chunks, uint64_t* d_total, void* d_ws, size_t ws_bytes,
+                    hipStream_t stream)
+{
+    if (chunks == 0) {
+        if (d_total) CSX_HIP(csx::launch_store_u64(d_total, 0, stream), "csx_scan total");
+        return ok();
+    }
+    if (!d_counts || !d_offsets) return fail(CSX_ERR_INVALID, "null counts or offsets pointer");
+    const ScanPlan p = plan_scan(chunks);
+    csx_status s;
+    if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
+    auto base = static_cast<uint8_t*>(d_ws);
+    CSX_HIP(csx::launch_scan(d_counts, d_offsets, chunks, d_total, reinterpret_cast<csx::WsHeader*>(base),
+                             reinterpret_cast<uint64_t*>(base + p.status_off), stream),
+            "csx_scan launch");
+    return ok();
+}
+
+csx_status csx_scan_tree(uint32_t* d_pss, uint64_t chunks, uint64_t* d_total, void* d_ws, size_t ws_bytes, hipStream_t stream)
+{
+    if (chunks == 0) {
+        if (d_total) CSX_HIP(csx::launch_store_u64(d_total, 0, stream), "csx_scan_tree total");
+        return ok();
+    }
+    if (!d_pss) return fail(CSX_ERR_INVALID, "null pss pointer");
+    const ScanPlan p = plan_scan(chunks);
+    csx_status s;
+    if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
+    auto base = static_cast<uint8_t*>(d_ws);
+    auto ex = reinterpret_cast<uint64_t*>(base + p.tmp_off);
+    CSX_HIP(csx::launch_scan(d_pss, ex, chunks, d_total, reinterpret_cast<csx::WsHeader*>(base), reinterpret_cast<uint64_t*>(base + p.status_off),
+                             stream),
+            "csx_scan_tree scan");
+    CSX_HIP(csx::launch_tree_from_offsets(d_pss, ex, chunks, stream), "csx_scan_tree tree");
+    return ok();
+}
+
+csx_status csx_tree_offsets(const uint32_t* d_tree, uint32_t* d_offsets, uint64_t chunks, hipStream_t stream)
+{
+    if (chunks == 0) return ok();
+    if (!d_tree || !d_offsets) return fail(CSX_ERR_INVALID, "null tree or offsets pointer");
+    if (chunks > (1ull << 40)) return fail(CSX_ERR_INVALID, "too many chunks");
+    CSX_HIP(csx::launch_tree_offsets(d_tree, d_offsets, chunks, stream), "csx_tree_offsets launch");
+    return ok();
+}
+
+csx_status csx_writeout(const void* d_in, const uint8_t* d_mask, const uint64_t* d_offsets, const uint32_t* d_counts, void* d_out, uint64_t n,
+                        uint32_t elem_size, uint32_t chunk_len, int bit_order, int variant, hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_width(elem_size)) || (s = check_chunk(chunk_len)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
+    if (variant < CSX_WRITEOUT_STAGED || variant > CSX_WRITEOUT_THREAD_PER_CHUNK) return fail(CSX_ERR_INVALID, "unknown writeout variant %d", variant);
+    if (n == 0) return ok();
+    if (!d_in || !d_mask || !d_offsets || !d_out) return fail(CSX_ERR_INVALID, "null input, mask, offsets or output pointer");
+    if (((uintptr_t)d_in % elem_size) || ((uintptr_t)d_out % elem_size))
+        return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
+    const uint32_t log2c = log2u(chunk_len);
+    auto in = static_cast<const uint8_t*>(d_in);
+    auto out = static_cast<uint8_t*>(d_out);
+    if (variant == CSX_WRITEOUT_THREAD_PER_CHUNK) {
+        CSX_HIP(timed(stream, [&] { return dispatch_tpc(elem_size, bit_order, in, d_mask, d_offsets, d_counts, out, n, log2c, stream); }),
+                "csx_writeout launch");
+        return ok();
+    }
+    const uint64_t tiles = (n + tile_elems(elem_size) - 1) / tile_elems(elem_size);
+    if (!launch_fits(tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)tiles);
+    csx::TileArgs a{};
+    a.in = in;
+    a.mask = d_mask;
+    a.out = out;
+    a.n = n;
+    a.offsets = d_offsets;
+    a.counts = d_counts;
+    a.log2c = log2c;
+    const bool aligned = (((uintptr_t)d_in | (uintptr_t)d_mask) & 15) == 0;
+    CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeWriteout, variant == CSX_WRITEOUT_STAGED, bit_order, aligned, a, stream); }),
+            "csx_writeout launch");
+    return ok();
+}
+
+#define CSX_WRITEOUT_W(W, T)                                                                                                                  \
+    csx_status csx_writeout_w##W(const T* d_in, const uint8_t* d_mask, const uint64_t* d_offsets, const uint32_t* d_counts, T* d_out,       \
+                                 uint64_t n, uint32_t chunk_len, int bit_order, int variant, hipStream_t stream)                              \
+    {                                                                                                                                         \
+        return csx_writeout(d_in, d_mask, d_offsets, d_counts, d_out, n, W, chunk_len, bit_order, variant, stream);                           \
+    }
+CSX_WRITEOUT_W(1, uint8_t)
+CSX_WRITEOUT_W(2, uint16_t)
+CSX_WRITEOUT_W(4, uint32_t)
+CSX_WRITEOUT_W(8, uint64_t)
+
+csx_status csx_workspace_size(uint64_t n, uint32_t elem_size, uint32_t chunk_len, size_t* bytes)
+{
+    csx_status s;
+    if ((s = check_width(elem_size)) || (s = check_chunk(chunk_len)) || (s = check_n(n))) return s;
+    if (!bytes) return fail(CSX_ERR_INVALID, "null size pointer");
+    *bytes = plan_compact(n, elem_size, chunk_len).total;
+    return ok();
+}
+
+static csx_status finish_host_count(uint64_t* h_count, const uint64_t* d_count_src, const csx::WsHeader* hdr, hipStream_t stream)
+{
+    uint32_t err = 0;
+    CSX_HIP(hipMemcpyAsync(h_count, d_count_src, sizeof(uint64_t), hipMemcpyDeviceToHost, stream), "count copy");
+    CSX_HIP(hipMemcpyAsync(&err, &hdr->error, sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "error copy");
+    CSX_HIP(hipStreamSynchronize(stream), "stream synchronize");
+    if (err) return fail(CSX_ERR_DEVICE, "a look-back wait gave up (error word %u)", err);
+    return ok();
+}
+
+csx_status csx_compact(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, uint32_t chunk_len, int bit_order,
+                       int pipeline, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_width(elem_size)) || (s = check_chunk(chunk_len)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
+    if (pipeline != CSX_PIPELINE_FUSED && pipeline != CSX_PIPELINE_THREE_PHASE) return fail(CSX_ERR_INVALID, "unknown pipeline %d", pipeline);
+    const CompactPlan p = plan_compact(n, elem_size, chunk_len);
+    if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
+    if (n > 0 && (!d_in || !d_mask || !d_out)) return fail(CSX_ERR_INVALID, "null input, mask or output pointer");
+    if (((uintptr_t)d_in % elem_size) || ((uintptr_t)d_out % elem_size))
+        return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
+    if (!launch_fits(p.fused_tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)p.fused_tiles);
+
+    auto base = static_cast<uint8_t*>(d_ws);
+    auto hdr = reinterpret_cast<csx::WsHeader*>(base);
+    auto status = reinterpret_cast<uint64_t*>(base + p.status_off);
+    uint64_t* count_dst = d_count ? d_count : &hdr->count;
+    auto in = static_cast<const uint8_t*>(d_in);
+    auto out = static_cast<uint8_t*>(d_out);
+    const bool aligned = (((uintptr_t)d_in | (uintptr_t)d_mask) & 15) == 0;
+
+    if (n == 0) {
+        CSX_HIP(csx::launch_lookback_init(hdr, status, 0, stream), "csx_compact init");
+        CSX_HIP(csx::launch_store_u64(count_dst, 0, stream), "csx_compact count");
+    }
+    else if (pipeline == CSX_PIPELINE_FUSED) {
+        CSX_HIP(csx::launch_lookback_init(hdr, status, p.fused_tiles, stream), "csx_compact init");
+        csx::TileArgs a{};
+        a.in = in;
+        a.mask = d_mask;
+        a.out = out;
+        a.n = n;
+        a.hdr = hdr;
+        a.status = status;
+        a.count_out = count_dst;
+        CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeFused, true, bit_order, aligned, a, stream); }),
+                "csx_compact fused");
+    }
+    else {
+        auto counts = reinterpret_cast<uint32_t*>(base + p.counts_off);
+        auto offsets = reinterpret_cast<uint64_t*>(base + p.offsets_off);
+        const uint32_t log2c = log2u(chunk_len);
+        CSX_HIP(csx::launch_popc(d_mask, counts, n, log2c, bit_order, stream), "csx_compact popc");
+        CSX_HIP(csx::launch_scan(counts, offsets, p.chunks, count_dst, hdr, status, stream), "csx_compact scan");
+        csx::TileArgs a{};
+        a.in = in;
+        a.mask = d_mask;
+        a.out = out;
+        a.n = n;
+        a.offsets = offsets;
+        a.counts = counts;
+        a.log2c = log2c;
+        CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeWriteout, true, bit_order, aligned, a, stream); }),
+                "csx_compact writeout");
+    }
+    if (h_count) return finish_host_count(h_count, count_dst, hdr, stream);
+    return ok();
+}
+
+#define CSX_COMPACT_W(W, T)                                                                                                                   \
+    csx_status csx_compact_w##W(const T* d_in, const uint8_t* d_mask, T* d_out, uint64_t n, uint32_t chunk_len, int bit_order, int pipeline, \
+                                uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream)                       \
+    {                                                                                                                                         \
+        return csx_compact(d_in, d_mask, d_out, n, W, chunk_len, bit_order, pipeline, d_count, h_count, d_ws, ws_bytes, stream);              \
+    }
+CSX_COMPACT_W(1, uint8_t)
+CSX_COMPACT_W(2, uint16_t)
+CSX_COMPACT_W(4, uint32_t)
+CSX_COMPACT_W(8, uint64_t)
+
+csx_status csx_workspace_error(const void* d_ws, uint32_t* h_error, hipStream_t stream)
+{
+    if (!d_ws || !h_error) return fail(CSX_ERR_INVALID, "null workspace or result pointer");
+    auto hdr = static_cast<const csx::WsHeader*>(d_ws);
+    CSX_HIP(hipMemcpyAsync(h_error, &hdr->error, sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "error copy");
+    CSX_HIP(hipStreamSynchronize(stream), "stream synchronize");
+    return ok();
+}
+
+csx_status csx_validate(const void* d_expected, const void* d_actual, uint64_t len, uint32_t elem_size, uint64_t* d_failures, hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_width(elem_size))) return s;
+    if (len == 0) return ok();
+    if (!d_expected || !d_actual || !d_failures) return fail(CSX_ERR_INVALID, "null pointer");
+    CSX_HIP(csx::launch_validate(d_expected, d_actual, len, elem_size, d_failures, stream), "csx_validate launch");
+    return ok();
+}
+
+csx_status csx_profile_begin(void)
+{
+    for (auto& p : g_prof.events) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    g_prof.events.clear();
+    g_prof.on = true;
+    return ok();
+}
+
+csx_status csx_profile_end(double* total_ms, uint64_t* launches)
+{
+    g_prof.on = false;
+    double sum = 0.0;
+    hipError_t err = hipSuccess;
+    for (auto& p : g_prof.events) {
+        float ms = 0.f;
+        if (err == hipSuccess) err = hipEventSynchronize(p.second);
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, p.first, p.second);
+        sum += ms;
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    const uint64_t count = g_prof.events.size();
+    g_prof.events.clear();
+    if (total_ms) *total_ms = sum;
+    if (launches) *launches = count;
+    if (err != hipSuccess) return hip_fail(err, "csx_profile_end");
+    return ok();
+}
+
+} // extern "C"

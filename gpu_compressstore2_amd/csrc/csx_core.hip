@@ -1,0 +1,93 @@
+// csx_core.hip -- launchers of the width-independent kernels: popcount,
+// look-back init, chunk-count scan, up-sweep tree, validation.
+#include "csx_internal.hpp"
+#include "csx_kernels.hpp"
+
+namespace csx {
+
+static unsigned grid_for(uint64_t items, uint64_t per_block, uint64_t cap)
+{
+    uint64_t g = (items + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+hipError_t launch_popc(const uint8_t* mask, uint32_t* counts, uint64_t n, uint32_t log2c, int order, hipStream_t s)
+{
+    const uint64_t nbytes = (n + 7) / 8;
+    const uint64_t nvec = (nbytes + 15) / 16;
+    const uint64_t chunks = (n + (1ull << log2c) - 1) >> log2c;
+    const bool aligned = ((uintptr_t)mask & 15) == 0;
+    unsigned grid;
+    if (log2c <= 13) {
+        grid = grid_for(nvec, kPopcBlock, 1u << 20);
+    }
+    else {
+        grid = grid_for(chunks, kPopcBlock / kWave, 1u << 20);
+    }
+    if (order == kBitMsb) {
+        if (aligned)
+            k_popc<kBitMsb, true><<<grid, kPopcBlock, 0, s>>>(mask, counts, n, log2c);
+        else
+            k_popc<kBitMsb, false><<<grid, kPopcBlock, 0, s>>>(mask, counts, n, log2c);
+    }
+    else {
+        if (aligned)
+            k_popc<kBitLsb, true><<<grid, kPopcBlock, 0, s>>>(mask, counts, n, log2c);
+        else
+            k_popc<kBitLsb, false><<<grid, kPopcBlock, 0, s>>>(mask, counts, n, log2c);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_lookback_init(WsHeader* hdr, uint64_t* status, uint64_t tiles, hipStream_t s)
+{
+    k_lookback_init<<<grid_for(tiles, 256, 1024), 256, 0, s>>>(hdr, status, tiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const uint32_t* counts, uint64_t* offsets, uint64_t m, uint64_t* total, WsHeader* hdr, uint64_t* status, hipStream_t s)
+{
+    const uint64_t tiles = (m + kScanTile - 1) / kScanTile;
+    hipError_t e = launch_lookback_init(hdr, status, tiles, s);
+    if (e != hipSuccess) return e;
+    k_scan<<<(unsigned)tiles, kScanBlock, 0, s>>>(counts, offsets, m, total, hdr, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_tree_from_offsets(uint32_t* pss, const uint64_t* ex, uint64_t m, hipStream_t s)
+{
+    k_tree_from_offsets<<<grid_for(m, 256, 1u << 16), 256, 0, s>>>(pss, ex, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_tree_offsets(const uint32_t* tree, uint32_t* out, uint64_t m, hipStream_t s)
+{
+    uint64_t p2 = 1;
+    while (p2 < m) p2 *= 2;
+    k_tree_offsets<<<grid_for(m, 256, 1u << 16), 256, 0, s>>>(tree, out, m, p2);
+    return hipGetLastError();
+}
+
+hipError_t launch_store_u64(uint64_t* dst, uint64_t v, hipStream_t s)
+{
+    k_store_u64<<<1, 64, 0, s>>>(dst, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_validate(const void* a, const void* b, uint64_t len, uint32_t w, uint64_t* failures, hipStream_t s)
+{
+    const unsigned grid = grid_for(len, 256, 4096);
+    auto pa = static_cast<const uint8_t*>(a);
+    auto pb = static_cast<const uint8_t*>(b);
+    switch (w) {
+        case 1: k_validate<1><<<grid, 256, 0, s>>>(pa, pb, len, failures); break;
+        case 2: k_validate<2><<<grid, 256, 0, s>>>(pa, pb, len, failures); break;
+        case 4: k_validate<4><<<grid, 256, 0, s>>>(pa, pb, len, failures); break;
+        default: k_validate<8><<<grid, 256, 0, s>>>(pa, pb, len, failures); break;
+    }
+    return hipGetLastError();
+}
+
+} // namespace csx

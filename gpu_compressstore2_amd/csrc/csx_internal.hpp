@@ -1,0 +1,37 @@
+// csx_internal.hpp -- host-side launchers behind the C ABI (not exported).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "csx_types.hpp"
+
+namespace csx {
+
+hipError_t launch_popc(const uint8_t* mask, uint32_t* counts, uint64_t n, uint32_t log2c, int order, hipStream_t s);
+hipError_t launch_lookback_init(WsHeader* hdr, uint64_t* status, uint64_t tiles, hipStream_t s);
+hipError_t launch_scan(const uint32_t* counts, uint64_t* offsets, uint64_t m, uint64_t* total, WsHeader* hdr, uint64_t* status, hipStream_t s);
+hipError_t launch_tree_from_offsets(uint32_t* pss, const uint64_t* ex, uint64_t m, hipStream_t s);
+hipError_t launch_tree_offsets(const uint32_t* tree, uint32_t* out, uint64_t m, hipStream_t s);
+hipError_t launch_store_u64(uint64_t* dst, uint64_t v, hipStream_t s);
+hipError_t launch_validate(const void* a, const void* b, uint64_t len, uint32_t w, uint64_t* failures, hipStream_t s);
+
+// tile engine, instantiated once per element width (csx_tile.hip)
+template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool aligned, const TileArgs& a, hipStream_t s);
+template <int W>
+hipError_t launch_thread_per_chunk(int order, const uint8_t* in, const uint8_t* mask, const uint64_t* offsets, const uint32_t* counts, uint8_t* out,
+                                   uint64_t n, uint32_t log2c, hipStream_t s);
+
+extern template hipError_t launch_tile<1>(int, bool, int, bool, const TileArgs&, hipStream_t);
+extern template hipError_t launch_tile<2>(int, bool, int, bool, const TileArgs&, hipStream_t);
+extern template hipError_t launch_tile<4>(int, bool, int, bool, const TileArgs&, hipStream_t);
+extern template hipError_t launch_tile<8>(int, bool, int, bool, const TileArgs&, hipStream_t);
+extern template hipError_t launch_thread_per_chunk<1>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
+                                                      uint32_t, hipStream_t);
+extern template hipError_t launch_thread_per_chunk<2>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
+                                                      uint32_t, hipStream_t);
+extern template hipError_t launch_thread_per_chunk<4>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
+                                                      uint32_t, hipStream_t);
+extern template hipError_t launch_thread_per_chunk<8>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
+                                                      uint32_t, hipStream_t);
+
+} // namespace csx

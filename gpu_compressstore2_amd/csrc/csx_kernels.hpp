@@ -1,0 +1,534 @@
+// csx_kernels.hpp -- gfx950 kernels of the three-phase bitmask compaction
+// pipeline and of the fused single-pass form.
+//
+// Reference path (RememberOfLife/gpu_compressstore2):
+//   phase 1 popcount   src/kernels/kernel_3pass.cuh:12-57
+//   phase 2 scan       src/kernels/kernel_3pass.cuh:59-214, CUB ExclusiveSum
+//                      (src/benchmarks.cuh:308,374,432)
+//   phase 3 writeout   src/kernels/kernel_3pass.cuh:216-693
+// The reference works in 32-lane warps over 1024-element strides; these
+// kernels are laid out for 64-lane waves, 16-byte loads and LDS staging.
+#pragma once
+
+#include "csx_device.hpp"
+
+namespace csx {
+
+// ======================================================================
+// phase 1: per-chunk popcount
+// ======================================================================
+//
+// Each thread owns one 16-byte mask vector (128 elements).  For a chunk of
+// C = 2^log2c elements:
+//   C = 32, 64      one vector holds 4 / 2 whole chunks
+//   128 <= C <= 8192  a chunk spans C/128 lanes of one wave: xor-reduce
+//   C > 8192        one wave per chunk, looping over its vectors
+// Bits at element index >= n do not count, whatever the buffer holds.
+constexpr int kPopcBlock = 256;
+
+template <int ORDER, bool ALIGNED>
+__global__ __launch_bounds__(kPopcBlock) void k_popc(const uint8_t* __restrict__ mask, uint32_t* __restrict__ counts, uint64_t n, uint32_t log2c)
+{
+    const uint64_t nbytes = (n + 7) / 8;
+    const uint64_t nvec = (nbytes + 15) / 16;
+    const uint64_t chunks = (n + (1ull << log2c) - 1) >> log2c;
+    const uint32_t lane = lane_id();
+    const uint64_t wave_global = (blockIdx.x * (uint64_t)kPopcBlock + threadIdx.x) / kWave;
+    const uint64_t waves_total = (uint64_t)gridDim.x * (kPopcBlock / kWave);
+
+    auto vec_words = [&](uint64_t v, uint32_t w[4]) {
+        u32x4 x = load_mask16<ALIGNED>(mask, v * 16, nbytes);
+        w[0] = to_lsb_word<ORDER>(x.x);
+        w[1] = to_lsb_word<ORDER>(x.y);
+        w[2] = to_lsb_word<ORDER>(x.z);
+        w[3] = to_lsb_word<ORDER>(x.w);
+        const uint64_t e0 = v * 128;
+        if (e0 + 128 > n) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint64_t b = e0 + 32 * k;
+                w[k] = b >= n ? 0u : low_bits(w[k], (uint32_t)(n - b));
+            }
+        }
+    };
+
+    if (log2c <= 6) {
+        // C = 32 or 64: whole chunks inside one vector
+        for (uint64_t v = blockIdx.x * (uint64_t)kPopcBlock + threadIdx.x; v < nvec; v += (uint64_t)gridDim.x * kPopcBlock) {
+            uint32_t w[4];
+            vec_words(v, w);
+            if (log2c == 5) {
+                const uint64_t c0 = v * 4;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (c0 + k < chunks) counts[c0 + k] = popc32(w[k]);
+            }
+            else {
+                const uint64_t c0 = v * 2;
+                if (c0 < chunks) counts[c0] = popc32(w[0]) + popc32(w[1]);
+                if (c0 + 1 < chunks) counts[c0 + 1] = popc32(w[2]) + popc32(w[3]);
+            }
+        }
+    }
+    else if (log2c <= 13) {
+        // C/128 lanes per chunk (1..64); loop bounds are uniform per wave
+        const uint32_t lanes_per_chunk = 1u << (log2c - 7);
+        for (uint64_t vb = wave_global * kWave; vb < nvec; vb += waves_total * kWave) {
+            const uint64_t v = vb + lane;
+            uint32_t cnt = 0;
+            if (v < nvec) {
+                uint32_t w[4];
+                vec_words(v, w);
+                cnt = popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
+            }
+            for (uint32_t d = lanes_per_chunk >> 1; d >= 1; d >>= 1) cnt += __shfl_xor(cnt, d, kWave);
+            if ((lane & (lanes_per_chunk - 1)) == 0 && v < nvec) {
+                const uint64_t c = v >> (log2c - 7);
+                if (c < chunks) counts[c] = cnt;
+            }
+        }
+    }
+    else {
+        // one wave per chunk
+        const uint64_t vecs_per_chunk = 1ull << (log2c - 7);
+        for (uint64_t c = wave_global; c < chunks; c += waves_total) {
+            uint32_t cnt = 0;
+            const uint64_t v0 = c * vecs_per_chunk;
+            for (uint64_t v = v0 + lane; v < v0 + vecs_per_chunk && v < nvec; v += kWave) {
+                uint32_t w[4];
+                vec_words(v, w);
+                cnt += popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
+            }
+            cnt = wave_sum(cnt);
+            if (lane == 0) counts[c] = cnt;
+        }
+    }
+}
+
+// ======================================================================
+// single-pass scans: shared init kernel
+// ======================================================================
+
+// Zero the ticket, the error word and `tiles` status words.  A plain kernel,
+// not a memset, so that streams captured into graphs replay it faithfully.
+static __global__ void k_lookback_init(WsHeader* hdr, uint64_t* status, uint64_t tiles)
+{
+    const uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (tid == 0) {
+        hdr->ticket = 0;
+        hdr->error = 0;
+    }
+    for (uint64_t i = tid; i < tiles; i += (uint64_t)gridDim.x * blockDim.x) status[i] = 0;
+}
+
+// ======================================================================
+// phase 2: exclusive scan of chunk counts (u32 -> u64), decoupled look-back
+// ======================================================================
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+static __global__ __launch_bounds__(kScanBlock) void k_scan(
+    const uint32_t* __restrict__ counts, uint64_t* __restrict__ offsets, uint64_t m, uint64_t* __restrict__ total, WsHeader* hdr,
+    uint64_t* status)
+{
+    __shared__ uint64_t s_scr[kScanBlock / kWave];
+    __shared__ uint64_t s_tile;
+    __shared__ uint64_t s_prefix;
+    if (threadIdx.x == 0) s_tile = atomicAdd(&hdr->ticket, 1u);
+    __syncthreads();
+    const uint64_t tile = s_tile;
+    const uint64_t ntiles = (m + kScanTile - 1) / kScanTile;
+    const uint64_t base = tile * kScanTile + threadIdx.x * (uint64_t)kScanItems;
+
+    uint32_t v[kScanItems];
+    if (base + kScanItems <= m && (((uintptr_t)(counts + base)) & 15) == 0) {
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; q++) {
+            u32x4 x = *reinterpret_cast<const u32x4*>(counts + base + 4 * q);
+            v[4 * q + 0] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            v[4 * q + 3] = x.w;
+        }
+    }
+    else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) v[k] = base + k < m ? counts[base + k] : 0u;
+    }
+    uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) sum += v[k];
+    uint64_t tile_total;
+    const uint64_t excl = block_exclusive_sum<kScanBlock>(sum, s_scr, &tile_total);
+
+    if (threadIdx.x < kWave) {
+        uint64_t pre = 0;
+        if (tile == 0) {
+            if (threadIdx.x == 0) publish_status(status, kFlagPrefix | tile_total);
+        }
+        else {
+            if (threadIdx.x == 0) publish_status(status + tile, kFlagAggregate | tile_total);
+            pre = lookback_exclusive(status, tile, &hdr->error);
+            if (threadIdx.x == 0) publish_status(status + tile, kFlagPrefix | (pre + tile_total));
+        }
+        if (threadIdx.x == 0) s_prefix = pre;
+    }
+    __syncthreads();
+    uint64_t run = s_prefix + excl;
+    if (base + kScanItems <= m) {
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) {
+            offsets[base + k] = run;
+            run += v[k];
+        }
+    }
+    else {
+        for (int k = 0; k < kScanItems; k++) {
+            if (base + k < m) offsets[base + k] = run;
+            run += v[k];
+        }
+    }
+    if (tile == ntiles - 1 && threadIdx.x == 0 && total) *total = s_prefix + tile_total;
+}
+
+// ======================================================================
+// phase 3 / fused: the compaction tile engine
+// ======================================================================
+//
+// A tile is TE consecutive elements handled by one 256-thread workgroup:
+//   * mask: TE/8 bytes, loaded as 16-byte vectors by the first TE/128
+//     threads, converted to LSB-first words and scanned (word popcounts) in
+//     LDS -> for every word the number of selected elements before it;
+//   * data: each thread owns VPT 16-byte vectors, striped across the block
+//     (vector j*256 + t) so every load instruction of a wave reads 1 KiB
+//     contiguous; a vector whose mask bits are all zero is not loaded;
+//   * the tile's output offset comes from the chunk offsets (three-phase
+//     writeout) or from a decoupled look-back over tile counts (fused);
+//   * STAGED: selected elements are written to LDS at their rank, shifted so
+//     LDS byte p maps to a 16-byte-aligned global address, then stored with
+//     16-byte stores (element stores only in the two boundary vectors that
+//     the neighbouring tiles share).  !STAGED: each selected element is
+//     stored straight from registers to out[offset + rank].
+template <int W> struct TileCfg {
+    static constexpr int BLOCK = 256;
+    static constexpr int VPT = 8;                    // 16-byte data vectors per thread
+    static constexpr int EPV = 16 / W;               // elements per vector
+    static constexpr int TE = BLOCK * VPT * EPV;     // elements per tile
+    static constexpr int NWORDS = TE / 32;           // 32-bit mask words per tile
+    static constexpr int MASK_THREADS = NWORDS / 4;  // threads loading 16 mask bytes
+    static constexpr int STAGE_BYTES = TE * W + 16;
+    static_assert(TE == kTileElems<W>, "tile size out of sync with the host planner");
+};
+
+template <int W> struct TileShared {
+    alignas(16) uint8_t stage[TileCfg<W>::STAGE_BYTES];
+    uint32_t words[TileCfg<W>::NWORDS];
+    uint32_t wpre[TileCfg<W>::NWORDS];
+    uint64_t scr[TileCfg<W>::BLOCK / kWave];
+    uint64_t tile;
+    uint64_t off;
+    uint32_t any;
+};
+
+template <int W> __device__ __forceinline__ uint64_t vec_elem(const u32x4& x, int k)
+{
+    if constexpr (W == 8) {
+        const uint32_t lo = k == 0 ? x.x : x.z;
+        const uint32_t hi = k == 0 ? x.y : x.w;
+        return ((uint64_t)hi << 32) | lo;
+    }
+    else {
+        constexpr int per = 4 / W;
+        const int comp = k / per;
+        const uint32_t word = comp == 0 ? x.x : comp == 1 ? x.y : comp == 2 ? x.z : x.w;
+        return (word >> (8 * W * (k % per))) & (W == 4 ? 0xFFFFFFFFu : ((1u << (8 * W)) - 1u));
+    }
+}
+
+template <int W> __device__ __forceinline__ void store_elem(uint8_t* p, uint64_t v)
+{
+    using T = typename word_t<W>::type;
+    *reinterpret_cast<T*>(p) = (T)v;
+}
+
+template <int W> __device__ __forceinline__ uint64_t load_elem(const uint8_t* p)
+{
+    using T = typename word_t<W>::type;
+    return (uint64_t)*reinterpret_cast<const T*>(p);
+}
+
+template <int W, int ORDER, bool ALIGNED, int MODE, bool STAGED>
+__global__ __launch_bounds__(256) void k_tile(TileArgs a)
+{
+    using C = TileCfg<W>;
+    __shared__ TileShared<W> sh;
+    const uint32_t t = threadIdx.x;
+    const uint64_t n = a.n;
+    const uint64_t nbytes = (n + 7) / 8;
+    const uint64_t ntiles = (n + C::TE - 1) / C::TE;
+
+    uint64_t tile;
+    if constexpr (MODE == kModeFused) {
+        if (t == 0) sh.tile = atomicAdd(&a.hdr->ticket, 1u);
+        __syncthreads();
+        tile = sh.tile;
+    }
+    else {
+        tile = blockIdx.x;
+    }
+    const uint64_t tbase = tile * (uint64_t)C::TE;
+    const bool partial = tbase + C::TE > n;
+
+    // ---- zero-chunk skipping (three-phase writeout with counts) ----
+    if constexpr (MODE == kModeWriteout) {
+        if (a.counts) {
+            const uint64_t chunks = (n + (1ull << a.log2c) - 1) >> a.log2c;
+            const uint64_t c0 = tbase >> a.log2c;
+            const uint64_t c1 = min(chunks, ((tbase + C::TE - 1) >> a.log2c) + 1);
+            int any = 0;
+            for (uint64_t c = c0 + t; c < c1; c += C::BLOCK) any |= a.counts[c] != 0;
+            if (!__syncthreads_or(any)) return;
+        }
+    }
+
+    // ---- mask words, word prefixes, tile count ----
+    uint32_t w[4] = {0, 0, 0, 0};
+    uint32_t wsum = 0;
+    if (t < (uint32_t)C::MASK_THREADS) {
+        const uint64_t off = tbase / 8 + 16ull * t;
+        u32x4 x = load_mask16<ALIGNED>(a.mask, off, nbytes);
+        w[0] = to_lsb_word<ORDER>(x.x);
+        w[1] = to_lsb_word<ORDER>(x.y);
+        w[2] = to_lsb_word<ORDER>(x.z);
+        w[3] = to_lsb_word<ORDER>(x.w);
+        if (partial) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint64_t b = tbase + 128ull * t + 32 * k;
+                w[k] = b >= n ? 0u : low_bits(w[k], (uint32_t)(n - b));
+            }
+        }
+        wsum = popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
+    }
+    uint64_t tile_count;
+    const uint64_t wex = block_exclusive_sum<C::BLOCK>((uint64_t)wsum, sh.scr, &tile_count);
+    if (t < (uint32_t)C::MASK_THREADS) {
+        uint32_t run = (uint32_t)wex;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            sh.words[4 * t + k] = w[k];
+            sh.wpre[4 * t + k] = run;
+            run += popc32(w[k]);
+        }
+    }
+    __syncthreads();
+
+    // ---- data loads (predicated on the vector's mask bits) ----
+    u32x4 vec[C::VPT];
+    uint32_t bits[C::VPT];
+    const uint8_t* tin = a.in + tbase * W;
+#pragma unroll
+    for (int j = 0; j < C::VPT; j++) {
+        const uint32_t v = j * C::BLOCK + t;
+        const uint32_t e0 = v * C::EPV;
+        bits[j] = (sh.words[e0 >> 5] >> (e0 & 31)) & (uint32_t)((1ull << C::EPV) - 1);
+        vec[j] = u32x4{0, 0, 0, 0};
+        if (bits[j]) {
+            if (ALIGNED && !partial) {
+                vec[j] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(tin + (uint64_t)v * 16));
+            }
+            else {
+                // element loads: unaligned base or the last (partial) tile;
+                // unselected elements (including all at index >= n) are not read
+                uint32_t parts[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int k = 0; k < C::EPV; k++) {
+                    if ((bits[j] >> k) & 1) {
+                        const uint64_t e = load_elem<W>(tin + ((uint64_t)e0 + k) * W);
+                        if constexpr (W == 8) {
+                            parts[2 * k] = (uint32_t)e;
+                            parts[2 * k + 1] = (uint32_t)(e >> 32);
+                        }
+                        else {
+                            constexpr int per = 4 / W;
+                            parts[k / per] |= (uint32_t)e << (8 * W * (k % per));
+                        }
+                    }
+                }
+                vec[j] = u32x4{parts[0], parts[1], parts[2], parts[3]};
+            }
+        }
+    }
+
+    // ---- tile output offset ----
+    if constexpr (MODE == kModeFused) {
+        if (t < kWave) {
+            uint64_t pre = 0;
+            if (tile == 0) {
+                if (t == 0) publish_status(a.status, kFlagPrefix | tile_count);
+            }
+            else {
+                if (t == 0) publish_status(a.status + tile, kFlagAggregate | tile_count);
+                pre = lookback_exclusive(a.status, tile, &a.hdr->error);
+                if (t == 0) publish_status(a.status + tile, kFlagPrefix | (pre + tile_count));
+            }
+            if (t == 0) {
+                sh.off = pre;
+                if (tile == ntiles - 1 && a.count_out) *a.count_out = pre + tile_count;
+            }
+        }
+        __syncthreads();
+    }
+    else {
+        const uint64_t chunk = tbase >> a.log2c;
+        const uint64_t cstart = chunk << a.log2c;
+        uint64_t extra = 0;
+        if (cstart < tbase) {
+            // chunk longer than a tile: count the chunk's selected elements before this tile
+            // (whole 16-byte vectors: tbase - cstart is a multiple of TE)
+            const uint64_t v0 = cstart / 128, v1 = tbase / 128;
+            uint32_t c = 0;
+            for (uint64_t v = v0 + t; v < v1; v += C::BLOCK) {
+                u32x4 x = load_mask16<ALIGNED>(a.mask, v * 16, nbytes);
+                c += popc32(x.x) + popc32(x.y) + popc32(x.z) + popc32(x.w);
+            }
+            uint64_t tot;
+            (void)block_exclusive_sum<C::BLOCK>((uint64_t)c, sh.scr, &tot);
+            extra = tot;
+        }
+        if (t == 0) sh.off = a.offsets[chunk] + extra;
+        __syncthreads();
+    }
+    const uint64_t toff = sh.off;
+
+    // ---- ranks, then staging (or direct stores) ----
+    const uintptr_t gstart = (uintptr_t)(a.out + toff * W);
+    const uint32_t shift = STAGED ? (uint32_t)(gstart & 15) : 0u;
+#pragma unroll
+    for (int j = 0; j < C::VPT; j++) {
+        if (!bits[j]) continue;
+        const uint32_t v = j * C::BLOCK + t;
+        const uint32_t e0 = v * C::EPV;
+        const uint32_t word = e0 >> 5;
+        uint32_t r = sh.wpre[word] + popc32(low_bits(sh.words[word], e0 & 31));
+#pragma unroll
+        for (int k = 0; k < C::EPV; k++) {
+            if ((bits[j] >> k) & 1) {
+                const uint64_t e = vec_elem<W>(vec[j], k);
+                if constexpr (STAGED) {
+                    store_elem<W>(sh.stage + shift + (uint64_t)r * W, e);
+                }
+                else {
+                    store_elem<W>(a.out + (toff + r) * W, e);
+                }
+                r++;
+            }
+        }
+    }
+    if constexpr (STAGED) {
+        __syncthreads();
+        const uint32_t bytes = shift + (uint32_t)tile_count * W;
+        const uint32_t q_end = (bytes + 15) / 16;
+        uint8_t* g0 = reinterpret_cast<uint8_t*>(gstart - shift);
+        for (uint32_t q = t; q < q_end; q += C::BLOCK) {
+            const uint32_t lo = q * 16, hi = lo + 16;
+            if (lo >= shift && hi <= bytes) {
+                const u32x4 x = *reinterpret_cast<const u32x4*>(sh.stage + lo);
+                __builtin_nontemporal_store(x, reinterpret_cast<u32x4*>(g0 + lo));
+            }
+            else {
+                // boundary vector shared with a neighbouring tile: own elements only
+                const uint32_t p0 = lo > shift ? lo : shift;
+                const uint32_t p1 = hi < bytes ? hi : bytes;
+                for (uint32_t p = p0; p < p1; p += W) store_elem<W>(g0 + p, load_elem<W>(sh.stage + p));
+            }
+        }
+    }
+}
+
+// ======================================================================
+// phase 3, thread-per-chunk variant (src/kernels/kernel_3pass.cuh:578-639)
+// ======================================================================
+template <int W, int ORDER>
+__global__ __launch_bounds__(256) void k_thread_per_chunk(
+    const uint8_t* __restrict__ in, const uint8_t* __restrict__ mask, const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ counts,
+    uint8_t* __restrict__ out, uint64_t n, uint32_t log2c)
+{
+    const uint64_t chunks = (n + (1ull << log2c) - 1) >> log2c;
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < chunks; c += (uint64_t)gridDim.x * blockDim.x) {
+        if (counts && counts[c] == 0) continue;
+        uint64_t o = offsets[c];
+        const uint64_t e_begin = c << log2c;
+        const uint64_t e_end = min(n, e_begin + (1ull << log2c));
+        for (uint64_t i = e_begin; i < e_end; i += 8) {
+            const uint32_t byte = mask[i / 8];
+            const uint32_t lim = (uint32_t)min<uint64_t>(8, e_end - i);
+            for (uint32_t b = 0; b < lim; b++) {
+                const uint32_t bit = ORDER == kBitMsb ? (byte >> (7 - b)) & 1u : (byte >> b) & 1u;
+                if (bit) {
+                    store_elem<W>(out + o * W, load_elem<W>(in + (i + b) * W));
+                    o++;
+                }
+            }
+        }
+    }
+}
+
+// Write a 64-bit value from the device (graph-safe replacement for a memset
+// or a host copy of a single word).
+static __global__ void k_store_u64(uint64_t* dst, uint64_t v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
+}
+
+// ======================================================================
+// the reference's up-sweep tree and its tree walk
+// ======================================================================
+
+// In place: pss[j] (a count on entry) becomes the sum of counts over
+// (j - lowbit(j+1), j], given the exclusive offsets `ex` of those counts.
+// Equals the state left by the log2(chunks) up-sweep launches of
+// src/kernels/kernel_3pass.cuh:69-153.
+static __global__ void k_tree_from_offsets(uint32_t* __restrict__ pss, const uint64_t* __restrict__ ex, uint64_t m)
+{
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < m; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t low = (j + 1) & (~(j + 1) + 1);
+        const uint64_t inclusive = ex[j] + pss[j];
+        pss[j] = (uint32_t)(inclusive - ex[j + 1 - low]);
+    }
+}
+
+// Exclusive offset of every chunk from the tree: the descent of
+// d_3pass_pproc_pssidx (src/kernels/kernel_3pass.cuh:155-169).
+static __global__ void k_tree_offsets(const uint32_t* __restrict__ tree, uint32_t* __restrict__ out, uint64_t m, uint64_t p2)
+{
+    for (uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; idx < m; idx += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t half = p2 / 2, consumed = 0;
+        uint32_t acc = 0;
+        while (half >= 1) {
+            if (idx >= consumed + half) {
+                acc += tree[consumed + half - 1];
+                consumed += half;
+            }
+            half /= 2;
+        }
+        out[idx] = acc;
+    }
+}
+
+// ======================================================================
+// bytewise comparison (src/kernels/data_generator.cuh:122-161)
+// ======================================================================
+template <int W>
+__global__ __launch_bounds__(256) void k_validate(const uint8_t* __restrict__ a, const uint8_t* __restrict__ b, uint64_t len, uint64_t* failures)
+{
+    uint64_t bad = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x) {
+        bad += load_elem<W>(a + i * W) != load_elem<W>(b + i * W);
+    }
+    bad = wave_sum(bad);
+    if (lane_id() == 0 && bad) atomicAdd((unsigned long long*)failures, (unsigned long long)bad);
+}
+
+} // namespace csx

@@ -1,0 +1,65 @@
+// csx_tile.hip -- tile-engine launchers for one element width.
+// Built once per width with -DCSX_TILE_W=1|2|4|8.
+#include "csx_internal.hpp"
+#include "csx_kernels.hpp"
+
+#ifndef CSX_TILE_W
+#error "define CSX_TILE_W (element width in bytes)"
+#endif
+
+namespace csx {
+
+template <int W, int ORDER, bool ALIGNED>
+static void launch_tile_impl(int mode, bool staged, const TileArgs& a, unsigned grid, hipStream_t s)
+{
+    if (mode == kModeFused) {
+        k_tile<W, ORDER, ALIGNED, kModeFused, true><<<grid, 256, 0, s>>>(a);
+    }
+    else if (staged) {
+        k_tile<W, ORDER, ALIGNED, kModeWriteout, true><<<grid, 256, 0, s>>>(a);
+    }
+    else {
+        k_tile<W, ORDER, ALIGNED, kModeWriteout, false><<<grid, 256, 0, s>>>(a);
+    }
+}
+
+template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool aligned, const TileArgs& a, hipStream_t s)
+{
+    const uint64_t tiles = (a.n + kTileElems<W> - 1) / kTileElems<W>;
+    if (tiles == 0) return hipSuccess;
+    const unsigned grid = (unsigned)tiles;
+    if (order == kBitMsb) {
+        if (aligned)
+            launch_tile_impl<W, kBitMsb, true>(mode, staged, a, grid, s);
+        else
+            launch_tile_impl<W, kBitMsb, false>(mode, staged, a, grid, s);
+    }
+    else {
+        if (aligned)
+            launch_tile_impl<W, kBitLsb, true>(mode, staged, a, grid, s);
+        else
+            launch_tile_impl<W, kBitLsb, false>(mode, staged, a, grid, s);
+    }
+    return hipGetLastError();
+}
+
+template <int W>
+hipError_t launch_thread_per_chunk(int order, const uint8_t* in, const uint8_t* mask, const uint64_t* offsets, const uint32_t* counts, uint8_t* out,
+                                   uint64_t n, uint32_t log2c, hipStream_t s)
+{
+    const uint64_t chunks = (n + (1ull << log2c) - 1) >> log2c;
+    uint64_t g = (chunks + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    if (order == kBitMsb)
+        k_thread_per_chunk<W, kBitMsb><<<(unsigned)g, 256, 0, s>>>(in, mask, offsets, counts, out, n, log2c);
+    else
+        k_thread_per_chunk<W, kBitLsb><<<(unsigned)g, 256, 0, s>>>(in, mask, offsets, counts, out, n, log2c);
+    return hipGetLastError();
+}
+
+template hipError_t launch_tile<CSX_TILE_W>(int, bool, int, bool, const TileArgs&, hipStream_t);
+template hipError_t launch_thread_per_chunk<CSX_TILE_W>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
+                                                        uint32_t, hipStream_t);
+
+} // namespace csx

@@ -1,0 +1,176 @@
+/*
+ * csx.h -- C ABI of the MI355X (gfx950) bitmask stream-compaction engine.
+ *
+ * Replaces the launch functions of RememberOfLife/gpu_compressstore2's
+ * three-phase pipeline (header-only CUDA templates, no library of its own):
+ *   phase 1  launch_3pass_popc_none        src/kernels/kernel_3pass.cuh:40-57
+ *   phase 2  launch_3pass_pss_gmem         src/kernels/kernel_3pass.cuh:107-153
+ *            launch_3pass_pss2_gmem        src/kernels/kernel_3pass.cuh:188-214
+ *            launch_cub_pss / CUB ExclusiveSum  src/cub_wraps.cuh:52-79,
+ *                                          src/benchmarks.cuh:308,374,432
+ *   phase 3  launch_3pass_proc_true<T,opt> src/kernels/kernel_3pass.cuh:536-576
+ *            launch_3pass_proc_none<T>     src/kernels/kernel_3pass.cuh:642-693
+ *   pipeline benchN(...)                   src/benchmarks.cuh:153-440
+ *
+ * Conventions (differences from the reference are deliberate):
+ *   - Every call is asynchronous on the caller's stream and returns a status
+ *     code instead of exiting (reference: CUDA_TRY -> exit,
+ *     src/cuda_try.cuh:8-20).
+ *   - The mask is const: bits at element index >= n are ignored in-kernel
+ *     instead of being cleared in the caller's buffer
+ *     (reference: prepare_buffers, src/benchmarks.cuh:109-121).
+ *   - Counts are 32-bit per chunk, offsets and totals 64-bit (the reference
+ *     uses u32 throughout, capping N below 2^32).  The returned selected
+ *     count is exact (the reference's CUB benches report twice the last
+ *     chunk's count, src/kernels/kernel_3pass.cuh:59-67).
+ *   - No allocation on any compute call: scratch lives in a caller-owned
+ *     workspace sized by the *_workspace_size queries (the two-call pattern
+ *     the reference uses for CUB, src/benchmarks.cuh:83-90).
+ *   - Element i of the mask is bit 7-(i%8) of byte i/8 with CSX_MSB_FIRST
+ *     (the reference's order, src/cub_wraps.cuh:20-24), bit i%8 with
+ *     CSX_LSB_FIRST.
+ *   - Elements are opaque 1/2/4/8-byte words (float32/float64 included): the
+ *     output is a bit-exact copy of the selected inputs in input order.
+ *   - Pointers to elements must be aligned to the element size.
+ */
+#ifndef CSX_H
+#define CSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CSX_OK = 0,
+    CSX_ERR_INVALID = 1,     /* bad argument (null pointer, size, chunk length, width, order) */
+    CSX_ERR_WORKSPACE = 2,   /* workspace missing or smaller than the size query returned */
+    CSX_ERR_HIP = 3,         /* a HIP runtime call failed; see csx_last_error() */
+    CSX_ERR_UNSUPPORTED = 4, /* valid request this build does not implement */
+    CSX_ERR_DEVICE = 5       /* a kernel reported an internal failure (csx_workspace_error) */
+} csx_status;
+
+typedef enum { CSX_MSB_FIRST = 0, CSX_LSB_FIRST = 1 } csx_bit_order;
+
+/* phase-3 writeout variants */
+typedef enum {
+    CSX_WRITEOUT_STAGED = 0,           /* ranks from mask words, LDS staging, 16-byte coalesced stores
+                                          (replaces the optimized writeout, kernel_3pass.cuh:329-467) */
+    CSX_WRITEOUT_DIRECT = 1,           /* same ranks, each element stored straight from registers
+                                          (replaces the naive writeout, kernel_3pass.cuh:216-327) */
+    CSX_WRITEOUT_THREAD_PER_CHUNK = 2  /* one thread walks one chunk bit by bit
+                                          (replaces proc_none, kernel_3pass.cuh:578-639) */
+} csx_writeout_variant;
+
+/* whole-pipeline forms of csx_compact */
+typedef enum {
+    CSX_PIPELINE_FUSED = 0,      /* one pass: mask popcount, decoupled look-back and staged writeout per tile */
+    CSX_PIPELINE_THREE_PHASE = 1 /* csx_popc -> csx_scan -> csx_writeout (STAGED), the reference's bench10 shape */
+} csx_pipeline;
+
+/* Largest element count accepted (offsets are 62-bit in the look-back status words). */
+#define CSX_MAX_ELEMENTS (UINT64_C(1) << 40)
+
+const char* csx_status_string(csx_status s);
+/* Message of the last failing call on this host thread ("" if none). */
+const char* csx_last_error(void);
+/* Version as major*10000 + minor*100 + patch. */
+int csx_version(void);
+
+/* Chunk lengths must be a power of two >= 32 (src/main.cu:236-245). */
+uint64_t csx_chunk_count(uint64_t n, uint32_t chunk_len);
+
+/* ---------------------------------------------------------------- phase 1
+ * d_counts[c] = number of selected elements in [c*chunk_len, min((c+1)*chunk_len, n)),
+ * for c < csx_chunk_count(n, chunk_len).
+ * Replaces launch_3pass_popc_none (src/kernels/kernel_3pass.cuh:40-57). */
+csx_status csx_popc(const uint8_t* d_mask, uint32_t* d_counts, uint64_t n, uint32_t chunk_len, int bit_order, hipStream_t stream);
+
+/* ---------------------------------------------------------------- phase 2
+ * d_offsets[c] = sum of d_counts[0..c) (exclusive), *d_total = sum of all
+ * (d_total nullable).  Single-pass decoupled look-back scan.
+ * Replaces launch_3pass_pss_gmem + launch_3pass_pss2_gmem
+ * (src/kernels/kernel_3pass.cuh:107-214) and the CUB ExclusiveSum +
+ * pssskip sequence (src/benchmarks.cuh:306-309). */
+csx_status csx_scan_workspace_size(uint64_t chunks, size_t* bytes);
+csx_status csx_scan(const uint32_t* d_counts, uint64_t* d_offsets, uint64_t chunks, uint64_t* d_total, void* d_ws, size_t ws_bytes,
+                    hipStream_t stream);
+
+/* The reference's in-place up-sweep tree (src/kernels/kernel_3pass.cuh:69-153):
+ * d_pss[j] becomes the sum of counts (j - lowbit(j+1), j]; *d_total (nullable)
+ * the exact sum.  d_pss is read as counts and overwritten.  Workspace as csx_scan. */
+csx_status csx_scan_tree(uint32_t* d_pss, uint64_t chunks, uint64_t* d_total, void* d_ws, size_t ws_bytes, hipStream_t stream);
+/* Exclusive offsets from that tree, the tree walk of d_3pass_pproc_pssidx
+ * (src/kernels/kernel_3pass.cuh:155-185). */
+csx_status csx_tree_offsets(const uint32_t* d_tree, uint32_t* d_offsets, uint64_t chunks, hipStream_t stream);
+
+/* ---------------------------------------------------------------- phase 3
+ * Writes the selected elements of d_in to d_out in input order, element
+ * d_offsets[c] + r for the r-th selected element of chunk c.
+ * d_counts (nullable) enables zero-chunk skipping: chunks whose count is 0
+ * are read neither from the mask nor from d_in (src/kernels/kernel_3pass.cuh:268-288).
+ * Replaces launch_3pass_proc_true<T, optimized> (kernel_3pass.cuh:536-576)
+ * and launch_3pass_proc_none<T> (kernel_3pass.cuh:642-693). */
+csx_status csx_writeout(const void* d_in, const uint8_t* d_mask, const uint64_t* d_offsets, const uint32_t* d_counts, void* d_out, uint64_t n,
+                        uint32_t elem_size, uint32_t chunk_len, int bit_order, int variant, hipStream_t stream);
+csx_status csx_writeout_w1(const uint8_t* d_in, const uint8_t* d_mask, const uint64_t* d_offsets, const uint32_t* d_counts, uint8_t* d_out,
+                           uint64_t n, uint32_t chunk_len, int bit_order, int variant, hipStream_t stream);
+csx_status csx_writeout_w2(const uint16_t* d_in, const uint8_t* d_mask, const uint64_t* d_offsets, const uint32_t* d_counts, uint16_t* d_out,
+                           uint64_t n, uint32_t chunk_len, int bit_order, int variant, hipStream_t stream);
+csx_status csx_writeout_w4(const uint32_t* d_in, const uint8_t* d_mask, const uint64_t* d_offsets, const uint32_t* d_counts, uint32_t* d_out,
+                           uint64_t n, uint32_t chunk_len, int bit_order, int variant, hipStream_t stream);
+csx_status csx_writeout_w8(const uint64_t* d_in, const uint8_t* d_mask, const uint64_t* d_offsets, const uint32_t* d_counts, uint64_t* d_out,
+                           uint64_t n, uint32_t chunk_len, int bit_order, int variant, hipStream_t stream);
+
+/* ---------------------------------------------------------------- pipeline
+ * Full compaction: d_out[0..k) = selected elements of d_in[0..n) in order,
+ * k written to *d_count (device, nullable) and, if h_count is not null, to
+ * *h_count after synchronising the stream.  d_out must hold k elements
+ * (n is always enough); nothing at or past d_out[k] is written.
+ * chunk_len only matters for CSX_PIPELINE_THREE_PHASE.
+ * Replaces the benchN compositions (src/benchmarks.cuh:153-440) -- bench10
+ * is CSX_PIPELINE_THREE_PHASE with the staged writeout. */
+csx_status csx_workspace_size(uint64_t n, uint32_t elem_size, uint32_t chunk_len, size_t* bytes);
+csx_status csx_compact(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, uint32_t chunk_len, int bit_order,
+                       int pipeline, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream);
+csx_status csx_compact_w1(const uint8_t* d_in, const uint8_t* d_mask, uint8_t* d_out, uint64_t n, uint32_t chunk_len, int bit_order,
+                          int pipeline, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream);
+csx_status csx_compact_w2(const uint16_t* d_in, const uint8_t* d_mask, uint16_t* d_out, uint64_t n, uint32_t chunk_len, int bit_order,
+                          int pipeline, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream);
+csx_status csx_compact_w4(const uint32_t* d_in, const uint8_t* d_mask, uint32_t* d_out, uint64_t n, uint32_t chunk_len, int bit_order,
+                          int pipeline, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream);
+csx_status csx_compact_w8(const uint64_t* d_in, const uint8_t* d_mask, uint64_t* d_out, uint64_t n, uint32_t chunk_len, int bit_order,
+                          int pipeline, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream);
+
+/* Non-zero if a single-pass kernel that used this workspace gave up waiting
+ * (never expected; reported as CSX_ERR_DEVICE by synchronous calls).
+ * Synchronises the stream. */
+csx_status csx_workspace_error(const void* d_ws, uint32_t* h_error, hipStream_t stream);
+
+/* ---------------------------------------------------------------- checks
+ * Number of positions i < len where a[i] and b[i] differ bytewise, added
+ * to *d_failures (device u64).  Replaces kernel_check_validation / validate
+ * (src/kernels/data_generator.cuh:122-161, src/benchmarks.cuh:442-450). */
+csx_status csx_validate(const void* d_expected, const void* d_actual, uint64_t len, uint32_t elem_size, uint64_t* d_failures,
+                        hipStream_t stream);
+
+/* ---------------------------------------------------------------- profiling
+ * Kernel timing for benchmark drivers: after csx_profile_begin(), every
+ * csx_compact / csx_writeout call on this host thread records a HIP event
+ * pair on its stream immediately around its dominant kernel (the fused tile
+ * kernel, or the writeout kernel of the three-phase pipeline).
+ * csx_profile_end() synchronises those events and returns the summed
+ * milliseconds and the number of timed launches.  (The reference times each
+ * phase with CUDA_TIME event pairs, src/cuda_time.cuh:11-21.) */
+csx_status csx_profile_begin(void);
+csx_status csx_profile_end(double* total_ms, uint64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CSX_H */

@@ -1,0 +1,110 @@
+"""CPU checks of the C ABI: the library loads, exports every symbol that
+include/csx.h declares, and rejects bad arguments before touching the GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import gpu_compressstore2_amd as csx
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for hdr in os.listdir(os.path.join(ROOT, "include")):
+        if hdr.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", hdr)).read()
+            names |= set(re.findall(r"^\s*(?:csx_status|const char\*|int|uint64_t)\s+(csx_\w+)\s*\(", text, re.M))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    for must in ("csx_popc", "csx_scan", "csx_writeout", "csx_compact", "csx_workspace_size", "csx_validate"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = csx.lib()
+    missing = [n for n in declared_symbols() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_signatures_cover_header():
+    assert set(declared_symbols()) <= set(csx.SIGNATURES)
+
+
+def test_status_strings_and_version():
+    L = csx.lib()
+    assert L.csx_status_string(0) == b"ok"
+    assert L.csx_status_string(2).startswith(b"workspace")
+    assert L.csx_version() >= 100
+    assert L.csx_chunk_count(1000, 32) == 32
+    assert L.csx_chunk_count(0, 1024) == 0
+
+
+@pytest.mark.parametrize("chunk", [0, 16, 31, 33, 100, 1000])
+def test_bad_chunk_length_rejected(chunk):
+    L = csx.lib()
+    sz = ctypes.c_size_t(0)
+    assert L.csx_workspace_size(1000, 4, chunk, ctypes.byref(sz)) == 1
+    assert b"power of two" in L.csx_last_error()
+    assert L.csx_popc(None, None, 1000, chunk, 0, None) == 1
+
+
+@pytest.mark.parametrize("w", [0, 3, 5, 16])
+def test_bad_width_rejected(w):
+    sz = ctypes.c_size_t(0)
+    assert csx.lib().csx_workspace_size(1000, w, 1024, ctypes.byref(sz)) == 1
+
+
+def test_bad_bit_order_and_variant_rejected():
+    L = csx.lib()
+    assert L.csx_popc(None, None, 10, 32, 2, None) == 1
+    assert L.csx_writeout(None, None, None, None, None, 10, 4, 32, 0, 7, None) == 1
+
+
+def test_too_many_elements_rejected():
+    sz = ctypes.c_size_t(0)
+    assert csx.lib().csx_workspace_size(csx.MAX_ELEMENTS + 1, 4, 1024, ctypes.byref(sz)) == 1
+
+
+def test_workspace_checks():
+    L = csx.lib()
+    need = csx.workspace_size(1 << 20, 4, 1024)
+    assert need > 0 and need % 256 == 0
+    # missing workspace
+    assert L.csx_compact(None, None, None, 1 << 20, 4, 1024, 0, 0, None, None, None, 0, None) == 2
+    # too small (a fake, never dereferenced pointer: the size check comes first)
+    assert L.csx_compact(None, None, None, 1 << 20, 4, 1024, 0, 0, None, None, ctypes.c_void_p(4096), need - 1, None) == 2
+    assert b"smaller" in L.csx_last_error()
+    # misaligned
+    assert L.csx_compact(None, None, None, 1 << 20, 4, 1024, 0, 0, None, None, ctypes.c_void_p(4097), need, None) == 2
+
+
+def test_null_data_pointers_rejected():
+    L = csx.lib()
+    need = csx.workspace_size(100, 4, 1024)
+    assert L.csx_compact(None, None, None, 100, 4, 1024, 0, 0, None, None, ctypes.c_void_p(4096), need, None) == 1
+    assert L.csx_writeout(None, None, None, None, None, 100, 4, 1024, 0, 0, None) == 1
+
+
+def test_unknown_pipeline_rejected():
+    need = csx.workspace_size(100, 4, 1024)
+    assert csx.lib().csx_compact(None, None, None, 100, 4, 1024, 0, 9, None, None, ctypes.c_void_p(4096), need, None) == 1
+
+
+def test_workspace_size_grows_with_n_and_shrinking_chunks():
+    a = csx.workspace_size(1 << 24, 4, 1024)
+    b = csx.workspace_size(1 << 26, 4, 1024)
+    c = csx.workspace_size(1 << 24, 4, 32)
+    assert b > a and c > a
+    assert csx.scan_workspace_size(0) >= 256
+
+
+def test_error_raises_python_exception():
+    with pytest.raises(csx.CsxError) as e:
+        csx.workspace_size(10, 3, 1024)
+    assert e.value.status == 1

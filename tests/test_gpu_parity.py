@@ -1,0 +1,279 @@
+"""GPU parity: every phase and pipeline of libcsx.so against the oracle,
+bit-exact, on seeded inputs (oracle sizes), plus full-size property checks."""
+import numpy as np
+import pytest
+import torch
+
+import gpu_compressstore2_amd as csx
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+NP = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
+VIEW = {1: np.uint8, 2: np.int16, 4: np.int32, 8: np.int64}
+TILE = {1: 32768, 2: 16384, 4: 8192, 8: 4096}
+
+
+def dev(a):
+    a = np.ascontiguousarray(a)
+    return torch.from_numpy(a.view(VIEW[a.dtype.itemsize])).cuda()
+
+
+def host(t, dtype):
+    return t.cpu().numpy().view(dtype)
+
+
+def make_mask(O, kind, n, seed, order):
+    if kind == "bern50":
+        return O.mask_bernoulli(n, 0.5, seed, order)
+    if kind == "bern01":
+        return O.mask_bernoulli(n, 0.01, seed, order)
+    if kind == "bern99":
+        return O.mask_bernoulli(n, 0.99, seed, order)
+    if kind == "zeros":
+        return np.zeros((n + 7) // 8, np.uint8)
+    if kind == "ones":
+        m = np.full((n + 7) // 8, 0xFF, np.uint8)
+        return m
+    if kind == "runs":
+        return O.mask_geometric_runs(n, 256, seed, order)
+    if kind == "zipf":
+        return O.mask_zipf_blocks(n, 4096, 1.2, 65, seed, order)
+    if kind == "periodic":
+        return O.mask_periodic(n, 0x9E3779B9, order)
+    raise ValueError(kind)
+
+
+def dirty_tail(mask, n, order):
+    """Set every bit past n in the last byte: the engine must ignore them."""
+    m = mask.copy()
+    if n % 8:
+        r = n % 8
+        m[-1] |= np.uint8(0xFF >> r) if order == oracle.MSB_FIRST else np.uint8((0xFF << r) & 0xFF)
+    return m
+
+
+def run_compact(data, mask, n, order, pipeline, chunk=1024, canary=True):
+    w = data.dtype.itemsize
+    d_in = dev(data)
+    d_mask = dev(mask)
+    # output with a canary pattern: nothing at or past out[count] may change
+    out_h = np.full(max(n, 1) + 64, 0xA5, np.uint8).view(np.uint8)
+    out_h = np.frombuffer(np.full((max(n, 1) + 64) * w, 0xA5, np.uint8).tobytes(), dtype=NP[w]).copy()
+    d_out = dev(out_h)
+    out, k = csx.compact(d_in, d_mask, n=n, chunk_len=chunk, bit_order=order, pipeline=pipeline, out=d_out)
+    torch.cuda.synchronize()
+    res = host(out, NP[w])
+    if canary:
+        assert (res[k:] == out_h[k:]).all(), "write past the selected count"
+    return res[:k], k
+
+
+CASES = [
+    (1, "bern50"), (7, "bern50"), (8, "ones"), (9, "bern50"), (100, "bern01"), (1000, "runs"),
+    ("T-1", "bern50"), ("T", "ones"), ("T+1", "bern99"), ("3T+17", "bern50"), ("3T+17", "zeros"),
+    (200003, "zipf"), (131072, "periodic"), ((1 << 20) + 5, "bern50"), ((1 << 20) + 5, "bern01"),
+]
+
+
+def resolve_n(n, w):
+    if isinstance(n, int):
+        return n
+    T = TILE[w]
+    return {"T-1": T - 1, "T": T, "T+1": T + 1, "3T+17": 3 * T + 17}[n]
+
+
+@pytest.mark.parametrize("pipeline", [csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE])
+@pytest.mark.parametrize("order", [oracle.MSB_FIRST, oracle.LSB_FIRST])
+@pytest.mark.parametrize("w", [1, 2, 4, 8])
+def test_compact_parity(O, w, order, pipeline):
+    for i, (n, kind) in enumerate(CASES):
+        n = resolve_n(n, w)
+        print(f"compact w={w} order={order} pipeline={pipeline} n={n} mask={kind}", flush=True)
+        data = O.fill_values(n, NP[w], seed=1000 + i)
+        mask = dirty_tail(make_mask(O, kind, n, 2000 + i, order), n, order)
+        expect, ke = O.compact(data, mask, n, order)
+        got, k = run_compact(data, mask, n, order, pipeline)
+        assert k == ke, (n, kind)
+        assert (got == expect).all(), (n, kind)
+
+
+@pytest.mark.parametrize("chunk", [32, 64, 256, 1024, 4096, 16384, 65536])
+def test_three_phase_chunk_lengths(O, chunk):
+    for w in (1, 4, 8):
+        for n in (chunk - 1, chunk * 5 + 3, 300007):
+            print(f"three-phase chunk={chunk} w={w} n={n}", flush=True)
+            data = O.fill_values(n, NP[w], seed=n)
+            mask = O.mask_bernoulli(n, 0.3, seed=n + 1)
+            expect, ke = O.compact(data, mask, n)
+            got, k = run_compact(data, mask, n, oracle.MSB_FIRST, csx.PIPELINE_THREE_PHASE, chunk=chunk)
+            assert k == ke and (got == expect).all()
+
+
+@pytest.mark.parametrize("order", [oracle.MSB_FIRST, oracle.LSB_FIRST])
+@pytest.mark.parametrize("chunk", [32, 64, 128, 256, 1024, 4096, 8192, 16384, 1 << 17])
+def test_popc_parity(O, chunk, order):
+    for n in (1, 7, 31, 32, 33, 127, 128, 129, chunk - 1, chunk, chunk + 1, 1000003):
+        print(f"popc chunk={chunk} order={order} n={n}", flush=True)
+        mask = dirty_tail(O.mask_bernoulli(n, 0.45, seed=n, bit_order=order), n, order)
+        expect = O.popc(mask, n, chunk, order)
+        got = host(csx.popc(dev(mask), n, chunk, order), np.uint32)
+        assert (got == expect).all()
+
+
+def test_popc_unaligned_mask(O):
+    n = 100000
+    base = O.mask_bernoulli(n + 64, 0.5, seed=5)
+    for off in (1, 3, 8):
+        mask = base[off:]
+        expect = O.popc(mask, n, 256)
+        d = dev(base)[off:]
+        got = host(csx.popc(d, n, 256), np.uint32)
+        assert (got == expect).all()
+
+
+@pytest.mark.parametrize("m", [1, 2, 4095, 4096, 4097, 100000, (1 << 20) + 3, 5 << 20])
+def test_scan_parity(O, m):
+    rng = np.random.default_rng(m)
+    counts = rng.integers(0, 4096, m).astype(np.uint32)
+    expect, total = O.exclusive_scan(counts)
+    offs, tot = csx.scan(dev(counts))
+    assert int(tot.item()) == total
+    assert (host(offs, np.uint64) == expect).all()
+
+
+def test_scan_large_values(O):
+    m = 300000
+    counts = np.full(m, 0xFFFFFFFF, np.uint32)
+    expect, total = O.exclusive_scan(counts)
+    offs, tot = csx.scan(dev(counts))
+    assert int(tot.item()) == total == m * 0xFFFFFFFF
+    assert (host(offs, np.uint64) == expect).all()
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 7, 64, 1000, 4097, 262144, 300001])
+def test_scan_tree_and_walk(O, m):
+    rng = np.random.default_rng(m + 1)
+    counts = rng.integers(0, 1024, m).astype(np.uint32)
+    tree_expect = O.upsweep_tree(counts)
+    d = dev(counts)
+    tree, tot = csx.scan_tree(d)
+    assert int(tot.item()) == int(counts.astype(np.int64).sum())
+    got_tree = host(tree, np.uint32)
+    assert (got_tree == tree_expect).all()
+    offs = host(csx.tree_offsets(tree), np.uint32)
+    sample = np.unique(np.concatenate([np.arange(min(m, 50)), rng.integers(0, m, 200)]))
+    for idx in sample:
+        assert offs[idx] == O.tree_prefix(tree_expect, int(idx))
+    csum = np.concatenate([[0], np.cumsum(counts.astype(np.int64))[:-1]]).astype(np.uint32)
+    assert (offs == csum).all()
+
+
+@pytest.mark.parametrize("variant", [csx.WRITEOUT_STAGED, csx.WRITEOUT_DIRECT, csx.WRITEOUT_THREAD_PER_CHUNK])
+@pytest.mark.parametrize("w", [1, 2, 4, 8])
+def test_writeout_variants(O, w, variant):
+    for chunk, n, kind, skip in ((32, 70001, "bern50", False), (1024, 250007, "zipf", True), (4096, 123457, "runs", True),
+                                 (65536, 300000, "bern01", True), (256, 9999, "zeros", True)):
+        for order in (oracle.MSB_FIRST, oracle.LSB_FIRST):
+            print(f"writeout w={w} variant={variant} chunk={chunk} n={n} mask={kind} skip={skip} order={order}", flush=True)
+            data = O.fill_values(n, NP[w], seed=n + w)
+            mask = dirty_tail(make_mask(O, kind, n, n + 9, order), n, order)
+            counts = O.popc(mask, n, chunk, order)
+            offs, total = O.exclusive_scan(counts)
+            expect, ke = O.compact(data, mask, n, order)
+            assert total == ke
+            canary = np.frombuffer(np.full((n + 16) * w, 0x5A, np.uint8).tobytes(), dtype=NP[w]).copy()
+            d_out = dev(canary)
+            csx.writeout(dev(data), dev(mask), dev(offs), n, chunk, order, variant, counts=dev(counts) if skip else None, out=d_out)
+            got = host(d_out, NP[w])
+            assert (got[:ke] == expect).all()
+            assert (got[ke:] == canary[ke:]).all()
+
+
+def test_unaligned_pointers(O):
+    n = 100003
+    for w in (1, 2, 4, 8):
+        for pipeline in (csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE):
+            data = O.fill_values(n + 8, NP[w], seed=w)
+            mask_full = O.mask_bernoulli((n + 64) * 1, 0.5, seed=w + 1)
+            for ioff, moff, ooff in ((1, 0, 0), (0, 1, 0), (0, 0, 1), (3, 5, 7)):
+                print(f"unaligned w={w} pipeline={pipeline} in+{ioff} mask+{moff} out+{ooff}", flush=True)
+                mask = mask_full[moff:]
+                expect, ke = O.compact(data[ioff:], mask, n)
+                d_in = dev(data)[ioff:]
+                d_mask = dev(mask_full)[moff:]
+                d_outbuf = torch.zeros(n + 16, dtype=dev(data).dtype, device="cuda")
+                d_out = d_outbuf[ooff:]
+                _, k = csx.compact(d_in, d_mask, n=n, pipeline=pipeline, out=d_out)
+                got = host(d_out, NP[w])[:k]
+                assert k == ke and (got == expect).all()
+
+
+def test_zero_elements(O):
+    d = torch.zeros(4, dtype=torch.int32, device="cuda")
+    m = torch.zeros(1, dtype=torch.uint8, device="cuda")
+    for pipeline in (csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE):
+        _, k = csx.compact(d, m, n=0, pipeline=pipeline)
+        assert k == 0
+
+
+def test_reference_generators_end_to_end(O):
+    """The reference's own inputs (src/main.cu:76, src/data_generator.cuh:14-188):
+    data bytes = uniform mask at p=0.5, seed 42; masks from each generator;
+    output checked against the restated generate_validation."""
+    n = 1 << 18
+    data = O.mask_uniform(n * 4, 0.5)[0].view(np.uint32)
+    masks = {
+        "uniform": O.mask_uniform(n // 8, 0.3)[0],
+        "zipf": O.mask_zipf(n // 8)[0],
+        "burst": O.mask_burst(n // 8, 0.01),
+        "offset": O.mask_offset(n // 8, 7),
+        "pattern": O.mask_pattern(n // 8, O.pattern_from_selectivity(8, 0.5), 8)[0],
+        "clustering": O.mask_clustering(0.5, 1, n)[0],
+    }
+    for name, mask in masks.items():
+        for pipeline in (csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE):
+            print(f"reference inputs mask={name} pipeline={pipeline}", flush=True)
+            expect, ke = O.generate_validation(data, mask)
+            got, k = run_compact(data, mask, n, oracle.MSB_FIRST, pipeline)
+            assert k == ke and (got == expect[:ke]).all()
+
+
+def test_validate_counts_mismatches(O):
+    a = np.arange(10000, dtype=np.uint32)
+    b = a.copy()
+    b[[0, 17, 9999]] += 1
+    f = csx.validate(dev(a), dev(b), 10000)
+    assert int(f.item()) == 3
+    f = csx.validate(dev(a), dev(a), 10000)
+    assert int(f.item()) == 0
+
+
+def test_repeat_is_deterministic(O):
+    n = 1 << 20
+    data = O.fill_values(n, np.uint32, seed=77)
+    mask = O.mask_bernoulli(n, 0.5, seed=78)
+    a, ka = run_compact(data, mask, n, oracle.MSB_FIRST, csx.PIPELINE_FUSED)
+    b, kb = run_compact(data, mask, n, oracle.MSB_FIRST, csx.PIPELINE_FUSED)
+    assert ka == kb and (a == b).all()
+
+
+@pytest.mark.parametrize("p", [0.01, 0.5, 0.99])
+@pytest.mark.parametrize("pipeline", [csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE])
+def test_full_size_2p28_index_identity(p, pipeline):
+    """2^28 u32 with data[i] = i: the output must be exactly the selected
+    indices, checked against torch.nonzero of the unpacked mask."""
+    n = 1 << 28
+    print(f"full size p={p} pipeline={pipeline}", flush=True)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(int(p * 1000) + pipeline)
+    bits = torch.rand(n, device="cuda", generator=g) < p
+    weights = torch.tensor([128, 64, 32, 16, 8, 4, 2, 1], dtype=torch.uint8, device="cuda")
+    mask = (bits.view(-1, 8).to(torch.uint8) * weights).sum(1, dtype=torch.uint8)
+    data = torch.arange(n, dtype=torch.int32, device="cuda")
+    out, k = csx.compact(data, mask, pipeline=pipeline)
+    expect = torch.nonzero(bits).flatten().to(torch.int32)
+    assert k == expect.numel()
+    assert torch.equal(out[:k], expect)
+    del bits, mask, data, out, expect
+    torch.cuda.empty_cache()
