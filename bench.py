@@ -4,14 +4,14 @@
 
 Workload: 2^28 four-byte elements (float32 uniform in [0,1), moved as opaque
 32-bit words) with packed MSB-first Bernoulli masks at 1 %, 50 % and 99 %
-selectivity.  A step is one csx_compact call (fused single-pass pipeline)
+selectivity.  A step is one csx_compact call (two-pass pipeline)
 over one input set, inputs already resident in HBM; three input sets rotate
 so the 256 MB Infinity Cache cannot serve a step from the previous one.
 
 Metric (effective GB/s) = (data bytes read + mask bytes read + selected bytes
 written) / time = (4N + N/8 + 4k) / t.  `value` is the 50 % point; the other
-selectivities, the three-phase pipeline and a plain device copy are reported
-under "sweep".  "roofline" prices the dominant kernel (the fused tile kernel)
+selectivities, the three-phase and fused pipelines and a plain device copy are reported
+under "sweep".  "roofline" prices the dominant kernel (the staged writeout)
 against the 8 TB/s HBM peak with its duration from HIP events recorded on its
 own stream; "cpu_baseline" times the reference's CPU compaction
 (generate_validation, compiled from its sources into oracle/_ref) on the host.
@@ -110,7 +110,7 @@ def main():
         masks, ks = zip(*[make_mask(n, p, 1000 * r + int(p * 100)) for r in range(ROTATE)])
         k_mean = sum(ks) / len(ks)
         step_bytes = 4 * n + n / 8 + 4 * k_mean
-        for name, pipeline in (("fused", csx.PIPELINE_FUSED), ("three_phase", csx.PIPELINE_THREE_PHASE)):
+        for name, pipeline in (("two_pass", csx.PIPELINE_TWO_PASS), ("three_phase", csx.PIPELINE_THREE_PHASE), ("fused", csx.PIPELINE_FUSED)):
 
             def step(i, pipeline=pipeline):
                 csx.compact(datas[i % ROTATE], masks[i % ROTATE], n=n, chunk_len=args.chunk, pipeline=pipeline, out=out, count=count,
@@ -142,7 +142,7 @@ def main():
                 print(f"[bench] MISMATCH {name} p={p}", file=sys.stderr)
                 sys.exit(1)
             del expect
-            if name == "fused" and p == 0.5:
+            if name == "two_pass" and p == 0.5:
                 main_result = (gbs, ms / K, kern_avg, step_bytes)
         del masks
 
@@ -153,7 +153,7 @@ def main():
     gbs, ms_step, kern_avg, step_bytes = main_result
     achieved = step_bytes / (kern_avg * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_fused_2p28_u32_p50.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_two_pass_2p28_u32_p50.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -164,7 +164,7 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "kernel": "csx::k_tile<4,MSB,aligned,fused,staged>",
+        "kernel": "csx::k_tile<4,MSB,aligned,two-pass,staged>",
         "algorithmic_bytes_per_launch": int(step_bytes),
     }
 
@@ -186,7 +186,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic: float32 uniform [0,1) moved as u32 words, seeded torch generators, MSB-first Bernoulli masks, 3 rotating input sets",
         "config": {
-            "workload": "single-column compaction of 2^28 4-byte elements, Bernoulli(0.5) mask, fused single-pass pipeline",
+            "workload": "single-column compaction of 2^28 4-byte elements, Bernoulli(0.5) mask, two-pass pipeline",
             "n": n,
             "elem_bytes": w,
             "selectivity": 0.5,

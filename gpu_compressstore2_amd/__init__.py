@@ -32,6 +32,7 @@ WRITEOUT_THREAD_PER_CHUNK = 2
 
 PIPELINE_FUSED = 0
 PIPELINE_THREE_PHASE = 1
+PIPELINE_TWO_PASS = 2
 
 MAX_ELEMENTS = 1 << 40
 
@@ -211,7 +212,7 @@ def writeout(inp, mask, offsets, n, chunk_len=1024, bit_order=MSB_FIRST, variant
     return out
 
 
-def compact(inp, mask, n=None, chunk_len=1024, bit_order=MSB_FIRST, pipeline=PIPELINE_FUSED, out=None, count=None, workspace=None,
+def compact(inp, mask, n=None, chunk_len=1024, bit_order=MSB_FIRST, pipeline=PIPELINE_TWO_PASS, out=None, count=None, workspace=None,
             stream=None, host_count=True):
     """Full compaction.  Returns (out, k) with k an int (host_count) or a device tensor."""
     _require_device(inp, mask, out, count)

@@ -115,9 +115,13 @@ ScanPlan plan_scan(uint64_t chunks)
 // Compaction workspace: header | status[max(fused tiles, scan tiles)] |
 // u32 counts[chunks] | u64 offsets[chunks].  The two pipelines share the
 // header and the status words.
+// header | status[max(fused tiles, scan tiles)] | u32 counts[chunks] |
+// u64 offsets[chunks] | u32 tile counts[tiles] | u64 group sums[groups].
+// The pipelines share the header and the status words.
 struct CompactPlan {
-    uint64_t fused_tiles, scan_tiles, chunks;
-    size_t status_off, counts_off, offsets_off, total;
+    uint64_t fused_tiles, scan_tiles, chunks, groups;
+    uint32_t log2g;
+    size_t status_off, counts_off, offsets_off, tcounts_off, gsums_off, total;
 };
 
 CompactPlan plan_compact(uint64_t n, uint32_t w, uint32_t chunk_len)
@@ -127,11 +131,15 @@ CompactPlan plan_compact(uint64_t n, uint32_t w, uint32_t chunk_len)
     p.fused_tiles = (n + te - 1) / te;
     p.chunks = chunk_count(n, chunk_len);
     p.scan_tiles = (p.chunks + csx::kScanTileElems - 1) / csx::kScanTileElems;
+    p.log2g = csx::two_pass_log2g(p.fused_tiles);
+    p.groups = (p.fused_tiles + (1ull << p.log2g) - 1) >> p.log2g;
     const uint64_t st = p.fused_tiles > p.scan_tiles ? p.fused_tiles : p.scan_tiles;
     p.status_off = sizeof(csx::WsHeader);
     p.counts_off = align_up(p.status_off + st * 8);
     p.offsets_off = align_up(p.counts_off + p.chunks * 4);
-    p.total = align_up(p.offsets_off + p.chunks * 8);
+    p.tcounts_off = align_up(p.offsets_off + p.chunks * 8);
+    p.gsums_off = align_up(p.tcounts_off + p.fused_tiles * 4);
+    p.total = align_up(p.gsums_off + p.groups * 8);
     return p;
 }
 
@@ -150,6 +158,17 @@ hipError_t dispatch_tile(uint32_t w, int mode, bool staged, int order, bool alig
         case 2: return csx::launch_tile<2>(mode, staged, order, aligned, a, s);
         case 4: return csx::launch_tile<4>(mode, staged, order, aligned, a, s);
         default: return csx::launch_tile<8>(mode, staged, order, aligned, a, s);
+    }
+}
+
+hipError_t dispatch_tile_counts(uint32_t w, int order, bool aligned, const uint8_t* mask, uint64_t n, uint32_t* tcounts, uint64_t* gsums,
+                                uint32_t log2g, hipStream_t s)
+{
+    switch (w) {
+        case 1: return csx::launch_tile_counts<1>(order, aligned, mask, n, tcounts, gsums, log2g, s);
+        case 2: return csx::launch_tile_counts<2>(order, aligned, mask, n, tcounts, gsums, log2g, s);
+        case 4: return csx::launch_tile_counts<4>(order, aligned, mask, n, tcounts, gsums, log2g, s);
+        default: return csx::launch_tile_counts<8>(order, aligned, mask, n, tcounts, gsums, log2g, s);
     }
 }
 
@@ -347,7 +366,8 @@ csx_status csx_compact(const void* d_in, const uint8_t* d_mask, void* d_out, uin
 {
     csx_status s;
     if ((s = check_width(elem_size)) || (s = check_chunk(chunk_len)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
-    if (pipeline != CSX_PIPELINE_FUSED && pipeline != CSX_PIPELINE_THREE_PHASE) return fail(CSX_ERR_INVALID, "unknown pipeline %d", pipeline);
+    if (pipeline != CSX_PIPELINE_FUSED && pipeline != CSX_PIPELINE_THREE_PHASE && pipeline != CSX_PIPELINE_TWO_PASS)
+        return fail(CSX_ERR_INVALID, "unknown pipeline %d", pipeline);
     const CompactPlan p = plan_compact(n, elem_size, chunk_len);
     if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
     if (n > 0 && (!d_in || !d_mask || !d_out)) return fail(CSX_ERR_INVALID, "null input, mask or output pointer");
@@ -380,12 +400,28 @@ csx_status csx_compact(const void* d_in, const uint8_t* d_mask, void* d_out, uin
         CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeFused, true, bit_order, aligned, a, stream); }),
                 "csx_compact fused");
     }
+    else if (pipeline == CSX_PIPELINE_TWO_PASS) {
+        auto tcounts = reinterpret_cast<uint32_t*>(base + p.tcounts_off);
+        auto gsums = reinterpret_cast<uint64_t*>(base + p.gsums_off);
+        CSX_HIP(dispatch_tile_counts(elem_size, bit_order, aligned, d_mask, n, tcounts, gsums, p.log2g, stream), "csx_compact tile counts");
+        csx::TileArgs a{};
+        a.in = in;
+        a.mask = d_mask;
+        a.out = out;
+        a.n = n;
+        a.count_out = count_dst;
+        a.tcounts = tcounts;
+        a.gsums = gsums;
+        a.log2g = p.log2g;
+        CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeTwoPass, true, bit_order, aligned, a, stream); }),
+                "csx_compact two-pass writeout");
+    }
     else {
         auto counts = reinterpret_cast<uint32_t*>(base + p.counts_off);
         auto offsets = reinterpret_cast<uint64_t*>(base + p.offsets_off);
         const uint32_t log2c = log2u(chunk_len);
-        CSX_HIP(csx::launch_popc(d_mask, counts, n, log2c, bit_order, stream), "csx_compact popc");
-        CSX_HIP(csx::launch_scan(counts, offsets, p.chunks, count_dst, hdr, status, stream), "csx_compact scan");
+        CSX_HIP(csx::launch_popc(d_mask, counts, n, log2c, bit_order, stream, hdr, status, p.scan_tiles), "csx_compact popc");
+        CSX_HIP(csx::launch_scan(counts, offsets, p.chunks, count_dst, hdr, status, stream, true), "csx_compact scan");
         csx::TileArgs a{};
         a.in = in;
         a.mask = d_mask;

@@ -13,7 +13,8 @@ static unsigned grid_for(uint64_t items, uint64_t per_block, uint64_t cap)
     return (unsigned)g;
 }
 
-hipError_t launch_popc(const uint8_t* mask, uint32_t* counts, uint64_t n, uint32_t log2c, int order, hipStream_t s)
+hipError_t launch_popc(const uint8_t* mask, uint32_t* counts, uint64_t n, uint32_t log2c, int order, hipStream_t s, WsHeader* hdr, uint64_t* status,
+                       uint64_t init_tiles)
 {
     const uint64_t nbytes = (n + 7) / 8;
     const uint64_t nvec = (nbytes + 15) / 16;
@@ -28,15 +29,15 @@ hipError_t launch_popc(const uint8_t* mask, uint32_t* counts, uint64_t n, uint32
     }
     if (order == kBitMsb) {
         if (aligned)
-            k_popc<kBitMsb, true><<<grid, kPopcBlock, 0, s>>>(mask, counts, n, log2c);
+            k_popc<kBitMsb, true><<<grid, kPopcBlock, 0, s>>>(mask, counts, n, log2c, hdr, status, init_tiles);
         else
-            k_popc<kBitMsb, false><<<grid, kPopcBlock, 0, s>>>(mask, counts, n, log2c);
+            k_popc<kBitMsb, false><<<grid, kPopcBlock, 0, s>>>(mask, counts, n, log2c, hdr, status, init_tiles);
     }
     else {
         if (aligned)
-            k_popc<kBitLsb, true><<<grid, kPopcBlock, 0, s>>>(mask, counts, n, log2c);
+            k_popc<kBitLsb, true><<<grid, kPopcBlock, 0, s>>>(mask, counts, n, log2c, hdr, status, init_tiles);
         else
-            k_popc<kBitLsb, false><<<grid, kPopcBlock, 0, s>>>(mask, counts, n, log2c);
+            k_popc<kBitLsb, false><<<grid, kPopcBlock, 0, s>>>(mask, counts, n, log2c, hdr, status, init_tiles);
     }
     return hipGetLastError();
 }
@@ -47,11 +48,14 @@ hipError_t launch_lookback_init(WsHeader* hdr, uint64_t* status, uint64_t tiles,
     return hipGetLastError();
 }
 
-hipError_t launch_scan(const uint32_t* counts, uint64_t* offsets, uint64_t m, uint64_t* total, WsHeader* hdr, uint64_t* status, hipStream_t s)
+hipError_t launch_scan(const uint32_t* counts, uint64_t* offsets, uint64_t m, uint64_t* total, WsHeader* hdr, uint64_t* status, hipStream_t s,
+                       bool initialized)
 {
     const uint64_t tiles = (m + kScanTile - 1) / kScanTile;
-    hipError_t e = launch_lookback_init(hdr, status, tiles, s);
-    if (e != hipSuccess) return e;
+    if (!initialized) {
+        hipError_t e = launch_lookback_init(hdr, status, tiles, s);
+        if (e != hipSuccess) return e;
+    }
     k_scan<<<(unsigned)tiles, kScanBlock, 0, s>>>(counts, offsets, m, total, hdr, status);
     return hipGetLastError();
 }

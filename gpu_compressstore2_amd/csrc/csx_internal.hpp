@@ -7,9 +7,15 @@
 
 namespace csx {
 
-hipError_t launch_popc(const uint8_t* mask, uint32_t* counts, uint64_t n, uint32_t log2c, int order, hipStream_t s);
+// With `hdr` set, the popcount kernel also initialises the look-back state of
+// `init_tiles` scan tiles (saves the init launch before launch_scan).
+hipError_t launch_popc(const uint8_t* mask, uint32_t* counts, uint64_t n, uint32_t log2c, int order, hipStream_t s, WsHeader* hdr = nullptr,
+                       uint64_t* status = nullptr, uint64_t init_tiles = 0);
+inline uint64_t scan_tiles(uint64_t m) { return (m + kScanTileElems - 1) / kScanTileElems; }
 hipError_t launch_lookback_init(WsHeader* hdr, uint64_t* status, uint64_t tiles, hipStream_t s);
-hipError_t launch_scan(const uint32_t* counts, uint64_t* offsets, uint64_t m, uint64_t* total, WsHeader* hdr, uint64_t* status, hipStream_t s);
+// initialized: the look-back state was already zeroed (by launch_popc)
+hipError_t launch_scan(const uint32_t* counts, uint64_t* offsets, uint64_t m, uint64_t* total, WsHeader* hdr, uint64_t* status, hipStream_t s,
+                       bool initialized = false);
 hipError_t launch_tree_from_offsets(uint32_t* pss, const uint64_t* ex, uint64_t m, hipStream_t s);
 hipError_t launch_tree_offsets(const uint32_t* tree, uint32_t* out, uint64_t m, hipStream_t s);
 hipError_t launch_store_u64(uint64_t* dst, uint64_t v, hipStream_t s);
@@ -18,6 +24,9 @@ hipError_t launch_validate(const void* a, const void* b, uint64_t len, uint32_t 
 // tile engine, instantiated once per element width (csx_tile.hip)
 template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool aligned, const TileArgs& a, hipStream_t s);
 template <int W>
+hipError_t launch_tile_counts(int order, bool aligned, const uint8_t* mask, uint64_t n, uint32_t* tcounts, uint64_t* gsums, uint32_t log2g,
+                              hipStream_t s);
+template <int W>
 hipError_t launch_thread_per_chunk(int order, const uint8_t* in, const uint8_t* mask, const uint64_t* offsets, const uint32_t* counts, uint8_t* out,
                                    uint64_t n, uint32_t log2c, hipStream_t s);
 
@@ -25,12 +34,16 @@ extern template hipError_t launch_tile<1>(int, bool, int, bool, const TileArgs&,
 extern template hipError_t launch_tile<2>(int, bool, int, bool, const TileArgs&, hipStream_t);
 extern template hipError_t launch_tile<4>(int, bool, int, bool, const TileArgs&, hipStream_t);
 extern template hipError_t launch_tile<8>(int, bool, int, bool, const TileArgs&, hipStream_t);
+extern template hipError_t launch_tile_counts<1>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, hipStream_t);
 extern template hipError_t launch_thread_per_chunk<1>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
                                                       uint32_t, hipStream_t);
+extern template hipError_t launch_tile_counts<2>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, hipStream_t);
 extern template hipError_t launch_thread_per_chunk<2>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
                                                       uint32_t, hipStream_t);
+extern template hipError_t launch_tile_counts<4>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, hipStream_t);
 extern template hipError_t launch_thread_per_chunk<4>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
                                                       uint32_t, hipStream_t);
+extern template hipError_t launch_tile_counts<8>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, hipStream_t);
 extern template hipError_t launch_thread_per_chunk<8>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
                                                       uint32_t, hipStream_t);
 

@@ -26,9 +26,20 @@ namespace csx {
 // Bits at element index >= n do not count, whatever the buffer holds.
 constexpr int kPopcBlock = 256;
 
+// When `hdr` is not null the kernel also zeroes the ticket, the error word
+// and `init_tiles` status words for a look-back scan launched after it.
 template <int ORDER, bool ALIGNED>
-__global__ __launch_bounds__(kPopcBlock) void k_popc(const uint8_t* __restrict__ mask, uint32_t* __restrict__ counts, uint64_t n, uint32_t log2c)
+__global__ __launch_bounds__(kPopcBlock) void k_popc(const uint8_t* __restrict__ mask, uint32_t* __restrict__ counts, uint64_t n, uint32_t log2c,
+                                                     WsHeader* hdr, uint64_t* status, uint64_t init_tiles)
 {
+    if (hdr) {
+        const uint64_t tid = blockIdx.x * (uint64_t)kPopcBlock + threadIdx.x;
+        if (tid == 0) {
+            hdr->ticket = 0;
+            hdr->error = 0;
+        }
+        for (uint64_t i = tid; i < init_tiles; i += (uint64_t)gridDim.x * kPopcBlock) status[i] = 0;
+    }
     const uint64_t nbytes = (n + 7) / 8;
     const uint64_t nvec = (nbytes + 15) / 16;
     const uint64_t chunks = (n + (1ull << log2c) - 1) >> log2c;
@@ -210,22 +221,21 @@ static __global__ __launch_bounds__(kScanBlock) void k_scan(
 //     16-byte stores (element stores only in the two boundary vectors that
 //     the neighbouring tiles share).  !STAGED: each selected element is
 //     stored straight from registers to out[offset + rank].
-template <int W> struct TileCfg {
+template <int W, int VPT_ = kTileVpt<W>> struct TileCfg {
     static constexpr int BLOCK = 256;
-    static constexpr int VPT = 8;                    // 16-byte data vectors per thread
+    static constexpr int VPT = VPT_;                 // 16-byte data vectors per thread
     static constexpr int EPV = 16 / W;               // elements per vector
     static constexpr int TE = BLOCK * VPT * EPV;     // elements per tile
     static constexpr int NWORDS = TE / 32;           // 32-bit mask words per tile
     static constexpr int MASK_THREADS = NWORDS / 4;  // threads loading 16 mask bytes
     static constexpr int STAGE_BYTES = TE * W + 16;
-    static_assert(TE == kTileElems<W>, "tile size out of sync with the host planner");
 };
 
-template <int W> struct TileShared {
-    alignas(16) uint8_t stage[TileCfg<W>::STAGE_BYTES];
-    uint32_t words[TileCfg<W>::NWORDS];
-    uint32_t wpre[TileCfg<W>::NWORDS];
-    uint64_t scr[TileCfg<W>::BLOCK / kWave];
+template <typename C> struct TileShared {
+    alignas(16) uint8_t stage[C::STAGE_BYTES];
+    uint32_t words[C::NWORDS];
+    uint32_t wpre[C::NWORDS];
+    uint64_t scr[C::BLOCK / kWave];
     uint64_t tile;
     uint64_t off;
     uint32_t any;
@@ -258,25 +268,17 @@ template <int W> __device__ __forceinline__ uint64_t load_elem(const uint8_t* p)
     return (uint64_t)*reinterpret_cast<const T*>(p);
 }
 
-template <int W, int ORDER, bool ALIGNED, int MODE, bool STAGED>
-__global__ __launch_bounds__(256) void k_tile(TileArgs a)
+// One tile: every thread of the block calls this with the same `tile`.
+// Contains __syncthreads; LDS in `sh` may be reused once all threads return
+// and pass a barrier.
+template <int W, int ORDER, bool ALIGNED, int MODE, bool STAGED, int VPT>
+__device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<W, VPT>>& sh, uint64_t tile)
 {
-    using C = TileCfg<W>;
-    __shared__ TileShared<W> sh;
+    using C = TileCfg<W, VPT>;
     const uint32_t t = threadIdx.x;
     const uint64_t n = a.n;
     const uint64_t nbytes = (n + 7) / 8;
     const uint64_t ntiles = (n + C::TE - 1) / C::TE;
-
-    uint64_t tile;
-    if constexpr (MODE == kModeFused) {
-        if (t == 0) sh.tile = atomicAdd(&a.hdr->ticket, 1u);
-        __syncthreads();
-        tile = sh.tile;
-    }
-    else {
-        tile = blockIdx.x;
-    }
     const uint64_t tbase = tile * (uint64_t)C::TE;
     const bool partial = tbase + C::TE > n;
 
@@ -311,8 +313,21 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a)
         }
         wsum = popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
     }
-    uint64_t tile_count;
-    const uint64_t wex = block_exclusive_sum<C::BLOCK>((uint64_t)wsum, sh.scr, &tile_count);
+    // two-pass: this thread's share of the selected count before the tile
+    uint64_t psum = 0;
+    if constexpr (MODE == kModeTwoPass) {
+        const uint64_t g = tile >> a.log2g;
+        for (uint64_t j = t; j < g; j += C::BLOCK) psum += a.gsums[j];
+        for (uint64_t j = (g << a.log2g) + t; j < tile; j += C::BLOCK) psum += a.tcounts[j];
+    }
+    // one block scan of (psum << 16 | wsum): the low field (a tile holds at
+    // most 32768 elements) never carries into the high one
+    uint64_t packed_total;
+    const uint64_t pex = block_exclusive_sum<C::BLOCK>((psum << 16) | wsum, sh.scr, &packed_total);
+    const uint64_t wex = pex & 0xFFFF;
+    const uint64_t tile_count = packed_total & 0xFFFF;
+    const uint64_t prefix_total = packed_total >> 16;
+    (void)prefix_total;
     if (t < (uint32_t)C::MASK_THREADS) {
         uint32_t run = (uint32_t)wex;
 #pragma unroll
@@ -362,7 +377,14 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a)
     }
 
     // ---- tile output offset ----
-    if constexpr (MODE == kModeFused) {
+    if constexpr (MODE == kModeTwoPass) {
+        if (t == 0) {
+            sh.off = prefix_total;
+            if (tile == ntiles - 1 && a.count_out) *a.count_out = prefix_total + tile_count;
+        }
+        __syncthreads();
+    }
+    else if constexpr (MODE == kModeFused) {
         if (t < kWave) {
             uint64_t pre = 0;
             if (tile == 0) {
@@ -445,6 +467,109 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a)
             }
         }
     }
+}
+
+// One tile per workgroup: tile = blockIdx.x (writeout) or a ticket (fused).
+template <int W, int ORDER, bool ALIGNED, int MODE, bool STAGED, int VPT = kTileVpt<W>>
+__global__ __launch_bounds__(256) void k_tile(TileArgs a)
+{
+    using C = TileCfg<W, VPT>;
+    __shared__ TileShared<C> sh;
+    uint64_t tile;
+    if constexpr (MODE == kModeFused) {
+        if (threadIdx.x == 0) sh.tile = atomicAdd(&a.hdr->ticket, 1u);
+        __syncthreads();
+        tile = sh.tile;
+    }
+    else {
+        tile = blockIdx.x;
+    }
+    tile_body<W, ORDER, ALIGNED, MODE, STAGED, VPT>(a, sh, tile);
+}
+
+// Persistent form: a grid sized to the machine walks all tiles, round-robin
+// (writeout) or by ticket (fused); the loop ends for every wave once the
+// tile index passes the last tile.
+template <int W, int ORDER, bool ALIGNED, int MODE, bool STAGED, int VPT = kTileVpt<W>>
+__global__ __launch_bounds__(256) void k_tile_loop(TileArgs a)
+{
+    using C = TileCfg<W, VPT>;
+    __shared__ TileShared<C> sh;
+    const uint64_t ntiles = (a.n + C::TE - 1) / C::TE;
+    if constexpr (MODE == kModeFused) {
+        while (true) {
+            if (threadIdx.x == 0) sh.tile = atomicAdd(&a.hdr->ticket, 1u);
+            __syncthreads();
+            const uint64_t tile = sh.tile;
+            __syncthreads();
+            if (tile >= ntiles) break;
+            tile_body<W, ORDER, ALIGNED, MODE, STAGED, VPT>(a, sh, tile);
+            __syncthreads();
+        }
+    }
+    else {
+        for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+            tile_body<W, ORDER, ALIGNED, MODE, STAGED, VPT>(a, sh, tile);
+            __syncthreads();
+        }
+    }
+}
+
+// ======================================================================
+// two-pass pipeline, pass 1: selected count of every tile and of every
+// group of 2^log2g consecutive tiles (one workgroup per group)
+// ======================================================================
+constexpr uint32_t kMaxGroupTiles = 8192;
+
+template <int W, int ORDER, bool ALIGNED>
+__global__ __launch_bounds__(256) void k_tile_counts(const uint8_t* __restrict__ mask, uint64_t n, uint32_t* __restrict__ tcounts,
+                                                     uint64_t* __restrict__ gsums, uint32_t log2g)
+{
+    constexpr uint64_t TE = kTileElems<W>;
+    constexpr uint32_t VEC_PER_TILE = (uint32_t)(TE / 128);  // 16-byte mask vectors per tile
+    constexpr uint32_t LANES_PER_TILE = VEC_PER_TILE < 64 ? VEC_PER_TILE : 64;
+    __shared__ uint32_t tc[kMaxGroupTiles];
+    __shared__ uint64_t scr[4];
+    const uint32_t t = threadIdx.x, lane = lane_id(), wave = t / kWave;
+    const uint64_t nbytes = (n + 7) / 8;
+    const uint64_t nvec = (nbytes + 15) / 16;
+    const uint64_t ntiles = (n + TE - 1) / TE;
+    const uint64_t G = 1ull << log2g;
+    const uint64_t tile0 = (uint64_t)blockIdx.x << log2g;
+    const uint64_t tile1 = min(ntiles, tile0 + G);
+    for (uint64_t j = t; j < G; j += 256) tc[j] = 0;
+    __syncthreads();
+    const uint64_t v0 = tile0 * VEC_PER_TILE;
+    const uint64_t v1 = min(nvec, tile1 * VEC_PER_TILE);
+    for (uint64_t vb = v0 + wave * kWave; vb < v1; vb += 256) {
+        const uint64_t v = vb + lane;
+        uint32_t c = 0;
+        if (v < v1) {
+            u32x4 x = load_mask16<ALIGNED>(mask, v * 16, nbytes);
+            uint32_t w[4] = {to_lsb_word<ORDER>(x.x), to_lsb_word<ORDER>(x.y), to_lsb_word<ORDER>(x.z), to_lsb_word<ORDER>(x.w)};
+            const uint64_t e0 = v * 128;
+            if (e0 + 128 > n) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint64_t b = e0 + 32 * k;
+                    w[k] = b >= n ? 0u : low_bits(w[k], (uint32_t)(n - b));
+                }
+            }
+            c = popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
+        }
+#pragma unroll
+        for (uint32_t d = LANES_PER_TILE / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
+        if ((lane & (LANES_PER_TILE - 1)) == 0 && v < v1 && c) atomicAdd(&tc[(v - v0) / VEC_PER_TILE], c);
+    }
+    __syncthreads();
+    uint64_t s = 0;
+    for (uint64_t j = t; j < tile1 - tile0; j += 256) {
+        tcounts[tile0 + j] = tc[j];
+        s += tc[j];
+    }
+    uint64_t total;
+    (void)block_exclusive_sum<256>(s, scr, &total);
+    if (t == 0) gsums[blockIdx.x] = total;
 }
 
 // ======================================================================

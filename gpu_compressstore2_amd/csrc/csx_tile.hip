@@ -12,7 +12,10 @@ namespace csx {
 template <int W, int ORDER, bool ALIGNED>
 static void launch_tile_impl(int mode, bool staged, const TileArgs& a, unsigned grid, hipStream_t s)
 {
-    if (mode == kModeFused) {
+    if (mode == kModeTwoPass) {
+        k_tile<W, ORDER, ALIGNED, kModeTwoPass, true><<<grid, 256, 0, s>>>(a);
+    }
+    else if (mode == kModeFused) {
         k_tile<W, ORDER, ALIGNED, kModeFused, true><<<grid, 256, 0, s>>>(a);
     }
     else if (staged) {
@@ -44,6 +47,28 @@ template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool a
 }
 
 template <int W>
+hipError_t launch_tile_counts(int order, bool aligned, const uint8_t* mask, uint64_t n, uint32_t* tcounts, uint64_t* gsums, uint32_t log2g,
+                              hipStream_t s)
+{
+    const uint64_t ntiles = (n + kTileElems<W> - 1) / kTileElems<W>;
+    const uint64_t groups = (ntiles + (1ull << log2g) - 1) >> log2g;
+    if (groups == 0) return hipSuccess;
+    if (order == kBitMsb) {
+        if (aligned)
+            k_tile_counts<W, kBitMsb, true><<<(unsigned)groups, 256, 0, s>>>(mask, n, tcounts, gsums, log2g);
+        else
+            k_tile_counts<W, kBitMsb, false><<<(unsigned)groups, 256, 0, s>>>(mask, n, tcounts, gsums, log2g);
+    }
+    else {
+        if (aligned)
+            k_tile_counts<W, kBitLsb, true><<<(unsigned)groups, 256, 0, s>>>(mask, n, tcounts, gsums, log2g);
+        else
+            k_tile_counts<W, kBitLsb, false><<<(unsigned)groups, 256, 0, s>>>(mask, n, tcounts, gsums, log2g);
+    }
+    return hipGetLastError();
+}
+
+template <int W>
 hipError_t launch_thread_per_chunk(int order, const uint8_t* in, const uint8_t* mask, const uint64_t* offsets, const uint32_t* counts, uint8_t* out,
                                    uint64_t n, uint32_t log2c, hipStream_t s)
 {
@@ -59,6 +84,7 @@ hipError_t launch_thread_per_chunk(int order, const uint8_t* in, const uint8_t* 
 }
 
 template hipError_t launch_tile<CSX_TILE_W>(int, bool, int, bool, const TileArgs&, hipStream_t);
+template hipError_t launch_tile_counts<CSX_TILE_W>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, hipStream_t);
 template hipError_t launch_thread_per_chunk<CSX_TILE_W>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
                                                         uint32_t, hipStream_t);
 

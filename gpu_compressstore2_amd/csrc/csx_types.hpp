@@ -14,14 +14,15 @@ struct WsHeader {
 };
 static_assert(sizeof(WsHeader) == 256, "header is one 256-byte block");
 
-// Elements per tile of the compaction tile engine: 256 threads x 8 vectors
-// of 16 bytes, i.e. 32 KiB of input per tile whatever the width.
-template <int W> constexpr uint64_t kTileElems = 256ull * 8ull * (16ull / W);
+// 16-byte input vectors per thread of the compaction tile engine, per width.
+template <int W> constexpr int kTileVpt = 8;
+// Elements per tile: 256 threads x kTileVpt vectors of 16 bytes.
+template <int W> constexpr uint64_t kTileElems = 256ull * kTileVpt<W> * (16ull / W);
 
 // Elements per tile of the chunk-count scan.
 constexpr uint64_t kScanTileElems = 256ull * 16ull;
 
-enum TileMode { kModeWriteout = 0, kModeFused = 1 };
+enum TileMode { kModeWriteout = 0, kModeFused = 1, kModeTwoPass = 2 };
 
 // Arguments of one tile-engine launch.
 struct TileArgs {
@@ -37,6 +38,20 @@ struct TileArgs {
     WsHeader* hdr;
     uint64_t* status;
     uint64_t* count_out;  // nullable: total selected count (device)
+    // two-pass: per-tile counts and per-group sums of 2^log2g tiles
+    const uint32_t* tcounts;
+    const uint64_t* gsums;
+    uint32_t log2g;
 };
+
+// Tiles per group of the two-pass pipeline: a power of two >= 16 keeping
+// the group count at most 2048 (the writeout sums at most 8 group sums per
+// thread), capped at 8192 tiles per group.
+inline uint32_t two_pass_log2g(uint64_t ntiles)
+{
+    uint32_t l = 4;
+    while (l < 13 && ((ntiles + (1ull << l) - 1) >> l) > 2048) l++;
+    return l;
+}
 
 } // namespace csx

@@ -58,7 +58,6 @@ def run_compact(data, mask, n, order, pipeline, chunk=1024, canary=True):
     d_in = dev(data)
     d_mask = dev(mask)
     # output with a canary pattern: nothing at or past out[count] may change
-    out_h = np.full(max(n, 1) + 64, 0xA5, np.uint8).view(np.uint8)
     out_h = np.frombuffer(np.full((max(n, 1) + 64) * w, 0xA5, np.uint8).tobytes(), dtype=NP[w]).copy()
     d_out = dev(out_h)
     out, k = csx.compact(d_in, d_mask, n=n, chunk_len=chunk, bit_order=order, pipeline=pipeline, out=d_out)
@@ -83,7 +82,10 @@ def resolve_n(n, w):
     return {"T-1": T - 1, "T": T, "T+1": T + 1, "3T+17": 3 * T + 17}[n]
 
 
-@pytest.mark.parametrize("pipeline", [csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE])
+PIPELINES = [csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE, csx.PIPELINE_TWO_PASS]
+
+
+@pytest.mark.parametrize("pipeline", PIPELINES)
 @pytest.mark.parametrize("order", [oracle.MSB_FIRST, oracle.LSB_FIRST])
 @pytest.mark.parametrize("w", [1, 2, 4, 8])
 def test_compact_parity(O, w, order, pipeline):
@@ -193,7 +195,7 @@ def test_writeout_variants(O, w, variant):
 def test_unaligned_pointers(O):
     n = 100003
     for w in (1, 2, 4, 8):
-        for pipeline in (csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE):
+        for pipeline in PIPELINES:
             data = O.fill_values(n + 8, NP[w], seed=w)
             mask_full = O.mask_bernoulli((n + 64) * 1, 0.5, seed=w + 1)
             for ioff, moff, ooff in ((1, 0, 0), (0, 1, 0), (0, 0, 1), (3, 5, 7)):
@@ -212,7 +214,7 @@ def test_unaligned_pointers(O):
 def test_zero_elements(O):
     d = torch.zeros(4, dtype=torch.int32, device="cuda")
     m = torch.zeros(1, dtype=torch.uint8, device="cuda")
-    for pipeline in (csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE):
+    for pipeline in PIPELINES:
         _, k = csx.compact(d, m, n=0, pipeline=pipeline)
         assert k == 0
 
@@ -232,7 +234,7 @@ def test_reference_generators_end_to_end(O):
         "clustering": O.mask_clustering(0.5, 1, n)[0],
     }
     for name, mask in masks.items():
-        for pipeline in (csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE):
+        for pipeline in PIPELINES:
             print(f"reference inputs mask={name} pipeline={pipeline}", flush=True)
             expect, ke = O.generate_validation(data, mask)
             got, k = run_compact(data, mask, n, oracle.MSB_FIRST, pipeline)
@@ -253,13 +255,25 @@ def test_repeat_is_deterministic(O):
     n = 1 << 20
     data = O.fill_values(n, np.uint32, seed=77)
     mask = O.mask_bernoulli(n, 0.5, seed=78)
-    a, ka = run_compact(data, mask, n, oracle.MSB_FIRST, csx.PIPELINE_FUSED)
-    b, kb = run_compact(data, mask, n, oracle.MSB_FIRST, csx.PIPELINE_FUSED)
-    assert ka == kb and (a == b).all()
+    for pipeline in PIPELINES:
+        a, ka = run_compact(data, mask, n, oracle.MSB_FIRST, pipeline)
+        b, kb = run_compact(data, mask, n, oracle.MSB_FIRST, pipeline)
+        assert ka == kb and (a == b).all()
+
+
+def test_two_pass_many_groups(O):
+    # more tiles than 2048 groups of 16: exercises larger groups and
+    # multi-load group sums in the writeout
+    n = (1 << 26) + 12345
+    data = O.fill_values(n, np.uint8, seed=3)
+    mask = O.mask_bernoulli(n, 0.2, seed=4)
+    expect, ke = O.compact(data, mask, n)
+    got, k = run_compact(data, mask, n, oracle.MSB_FIRST, csx.PIPELINE_TWO_PASS, canary=False)
+    assert k == ke and (got == expect).all()
 
 
 @pytest.mark.parametrize("p", [0.01, 0.5, 0.99])
-@pytest.mark.parametrize("pipeline", [csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE])
+@pytest.mark.parametrize("pipeline", PIPELINES)
 def test_full_size_2p28_index_identity(p, pipeline):
     """2^28 u32 with data[i] = i: the output must be exactly the selected
     indices, checked against torch.nonzero of the unpacked mask."""
