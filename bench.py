@@ -85,7 +85,12 @@ def main():
     ap.add_argument("--chunk", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline")
+    ap.add_argument("--sels", default="0.01,0.5,0.99", help="selectivities to sweep (0.5 is always included)")
+    ap.add_argument("--pipelines", default="two_pass,three_phase,fused", help="pipelines to sweep (two_pass is always included)")
     args = ap.parse_args()
+    sels = sorted(set(float(x) for x in args.sels.split(",")) | {0.5})
+    all_pipes = {"two_pass": csx.PIPELINE_TWO_PASS, "three_phase": csx.PIPELINE_THREE_PHASE, "fused": csx.PIPELINE_FUSED}
+    pipes = [(k, all_pipes[k]) for k in all_pipes if k == "two_pass" or k in args.pipelines.split(",")]
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the compaction shards "
@@ -106,11 +111,11 @@ def main():
 
     sweep = {}
     main_result = None
-    for p in (0.01, 0.5, 0.99):
+    for p in sels:
         masks, ks = zip(*[make_mask(n, p, 1000 * r + int(p * 100)) for r in range(ROTATE)])
         k_mean = sum(ks) / len(ks)
         step_bytes = 4 * n + n / 8 + 4 * k_mean
-        for name, pipeline in (("two_pass", csx.PIPELINE_TWO_PASS), ("three_phase", csx.PIPELINE_THREE_PHASE), ("fused", csx.PIPELINE_FUSED)):
+        for name, pipeline in pipes:
 
             def step(i, pipeline=pipeline):
                 csx.compact(datas[i % ROTATE], masks[i % ROTATE], n=n, chunk_len=args.chunk, pipeline=pipeline, out=out, count=count,

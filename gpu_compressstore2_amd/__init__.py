@@ -84,6 +84,8 @@ SIGNATURES = {
     "csx_compact_w8": (_int, [_vp, _vp, _vp, _u64, _u32, _int, _int, _vp, _vp, _vp, _sz, _vp]),
     "csx_workspace_error": (_int, [_vp, ctypes.POINTER(_u32), _vp]),
     "csx_validate": (_int, [_vp, _vp, _u64, _u32, _vp, _vp]),
+    "csx_scan_u32": (_int, [_vp, _vp, _u64, _vp, _vp, _sz, _vp]),
+    "csx_writeout_ex": (_int, [_vp, _vp]),
     "csx_profile_begin": (_int, []),
     "csx_profile_end": (_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
 }
@@ -199,17 +201,42 @@ def tree_offsets(tree, out=None, stream=None):
     return out[:m]
 
 
-def writeout(inp, mask, offsets, n, chunk_len=1024, bit_order=MSB_FIRST, variant=WRITEOUT_STAGED, counts=None, out=None, stream=None):
-    """Phase 3: selected elements of inp to out at the chunk offsets."""
-    _require_device(inp, mask, offsets, counts, out)
+class WriteoutArgs(ctypes.Structure):
+    """Layout of csx_writeout_args."""
+
+    _fields_ = [
+        ("input", _vp), ("mask", _vp), ("out", _vp), ("offsets", _vp), ("offsets_u32", _int), ("counts", _vp), ("out_base", _vp),
+        ("n", _u64), ("elem_size", _u32), ("chunk_len", _u32), ("bit_order", _int), ("variant", _int),
+    ]
+
+
+def writeout(inp, mask, offsets, n, chunk_len=1024, bit_order=MSB_FIRST, variant=WRITEOUT_STAGED, counts=None, out=None, stream=None,
+             out_base=None):
+    """Phase 3: selected elements of inp to out at the chunk offsets (int64
+    offsets, or int32 offsets as the reference keeps them); out_base is an
+    optional one-element device tensor added to every output index."""
+    _require_device(inp, mask, offsets, counts, out, out_base)
     if out is None:
         out = torch.empty(max(n, 1), dtype=inp.dtype, device=inp.device)
-    _check(
-        lib().csx_writeout(
-            _ptr(inp), _ptr(mask), _ptr(offsets), _ptr(counts), _ptr(out), n, inp.element_size(), chunk_len, bit_order, variant, _stream(stream)
-        )
-    )
+    wa = WriteoutArgs(_ptr(inp), _ptr(mask), _ptr(out), _ptr(offsets), int(offsets.element_size() == 4), _ptr(counts), _ptr(out_base),
+                      n, inp.element_size(), chunk_len, bit_order, variant)
+    _check(lib().csx_writeout_ex(ctypes.byref(wa), _stream(stream)))
     return out
+
+
+def scan_u32(counts, offsets=None, total=None, workspace=None, stream=None):
+    """Exclusive scan into 32-bit offsets (offsets may be counts itself)."""
+    _require_device(counts)
+    m = counts.numel()
+    dev = counts.device
+    if offsets is None:
+        offsets = torch.empty(max(m, 1), dtype=torch.int32, device=dev)
+    if total is None:
+        total = torch.empty(1, dtype=torch.int64, device=dev)
+    if workspace is None:
+        workspace = alloc_workspace(scan_workspace_size(m), dev)
+    _check(lib().csx_scan_u32(_ptr(counts), _ptr(offsets), m, _ptr(total), _ptr(workspace), workspace.numel(), _stream(stream)))
+    return offsets[:m], total
 
 
 def compact(inp, mask, n=None, chunk_len=1024, bit_order=MSB_FIRST, pipeline=PIPELINE_TWO_PASS, out=None, count=None, workspace=None,

@@ -172,14 +172,13 @@ hipError_t dispatch_tile_counts(uint32_t w, int order, bool aligned, const uint8
     }
 }
 
-hipError_t dispatch_tpc(uint32_t w, int order, const uint8_t* in, const uint8_t* mask, const uint64_t* offsets, const uint32_t* counts,
-                        uint8_t* out, uint64_t n, uint32_t log2c, hipStream_t s)
+hipError_t dispatch_tpc(uint32_t w, int order, const csx::TileArgs& a, hipStream_t s)
 {
     switch (w) {
-        case 1: return csx::launch_thread_per_chunk<1>(order, in, mask, offsets, counts, out, n, log2c, s);
-        case 2: return csx::launch_thread_per_chunk<2>(order, in, mask, offsets, counts, out, n, log2c, s);
-        case 4: return csx::launch_thread_per_chunk<4>(order, in, mask, offsets, counts, out, n, log2c, s);
-        default: return csx::launch_thread_per_chunk<8>(order, in, mask, offsets, counts, out, n, log2c, s);
+        case 1: return csx::launch_thread_per_chunk<1>(order, a, s);
+        case 2: return csx::launch_thread_per_chunk<2>(order, a, s);
+        case 4: return csx::launch_thread_per_chunk<4>(order, a, s);
+        default: return csx::launch_thread_per_chunk<8>(order, a, s);
     }
 }
 
@@ -269,7 +268,25 @@ csx_status csx_scan(const uint32_t* d_counts, uint64_t* d_offsets, uint64_t chun
     return ok();
 }
 
-csx_status csx_scan_tree(uint32_t* d_pss, uint64_t chunks, uint64_t* d_total, void* d_ws, size_t ws_bytes, hipStream_t stream)
+csx_status csx_scan_u32(const uint32_t* d_counts, uint32_t* d_offsets, uint64_t chunks, uint64_t* d_total, void* d_ws, size_t ws_bytes,
+                        hipStream_t stream)
+{
+    if (chunks == 0) {
+        if (d_total) CSX_HIP(csx::launch_store_u64(d_total, 0, stream), "csx_scan_u32 total");
+        return ok();
+    }
+    if (!d_counts || !d_offsets) return fail(CSX_ERR_INVALID, "null counts or offsets pointer");
+    const ScanPlan p = plan_scan(chunks);
+    csx_status s;
+    if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
+    auto base = static_cast<uint8_t*>(d_ws);
+    CSX_HIP(csx::launch_scan32(d_counts, d_offsets, chunks, d_total, reinterpret_cast<csx::WsHeader*>(base),
+                               reinterpret_cast<uint64_t*>(base + p.status_off), stream),
+            "csx_scan_u32 launch");
+    return ok();
+}
+
+csx_status csx_scan_tree(uint32_t* d_pss,uint64_t chunks, uint64_t* d_total, void* d_ws, size_t ws_bytes, hipStream_t stream)
 {
     if (chunks == 0) {
         if (d_total) CSX_HIP(csx::launch_store_u64(d_total, 0, stream), "csx_scan_tree total");
@@ -297,38 +314,60 @@ csx_status csx_tree_offsets(const uint32_t* d_tree, uint32_t* d_offsets, uint64_
     return ok();
 }
 
-csx_status csx_writeout(const void* d_in, const uint8_t* d_mask, const uint64_t* d_offsets, const uint32_t* d_counts, void* d_out, uint64_t n,
-                        uint32_t elem_size, uint32_t chunk_len, int bit_order, int variant, hipStream_t stream)
+csx_status csx_writeout_ex(const csx_writeout_args* wa, hipStream_t stream)
 {
+    if (!wa) return fail(CSX_ERR_INVALID, "null argument block");
+    const uint32_t elem_size = wa->elem_size, chunk_len = wa->chunk_len;
+    const uint64_t n = wa->n;
+    const int variant = wa->variant, bit_order = wa->bit_order;
     csx_status s;
     if ((s = check_width(elem_size)) || (s = check_chunk(chunk_len)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
     if (variant < CSX_WRITEOUT_STAGED || variant > CSX_WRITEOUT_THREAD_PER_CHUNK) return fail(CSX_ERR_INVALID, "unknown writeout variant %d", variant);
     if (n == 0) return ok();
-    if (!d_in || !d_mask || !d_offsets || !d_out) return fail(CSX_ERR_INVALID, "null input, mask, offsets or output pointer");
-    if (((uintptr_t)d_in % elem_size) || ((uintptr_t)d_out % elem_size))
+    if (!wa->in || !wa->mask || !wa->offsets || !wa->out) return fail(CSX_ERR_INVALID, "null input, mask, offsets or output pointer");
+    if (((uintptr_t)wa->in % elem_size) || ((uintptr_t)wa->out % elem_size))
         return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
-    const uint32_t log2c = log2u(chunk_len);
-    auto in = static_cast<const uint8_t*>(d_in);
-    auto out = static_cast<uint8_t*>(d_out);
+    csx::TileArgs a{};
+    a.in = static_cast<const uint8_t*>(wa->in);
+    a.mask = wa->mask;
+    a.out = static_cast<uint8_t*>(wa->out);
+    a.n = n;
+    if (wa->offsets_u32)
+        a.offsets32 = static_cast<const uint32_t*>(wa->offsets);
+    else
+        a.offsets = static_cast<const uint64_t*>(wa->offsets);
+    a.counts = wa->counts;
+    a.log2c = log2u(chunk_len);
+    a.out_base = wa->out_base;
     if (variant == CSX_WRITEOUT_THREAD_PER_CHUNK) {
-        CSX_HIP(timed(stream, [&] { return dispatch_tpc(elem_size, bit_order, in, d_mask, d_offsets, d_counts, out, n, log2c, stream); }),
-                "csx_writeout launch");
+        CSX_HIP(timed(stream, [&] { return dispatch_tpc(elem_size, bit_order, a, stream); }), "csx_writeout launch");
         return ok();
     }
     const uint64_t tiles = (n + tile_elems(elem_size) - 1) / tile_elems(elem_size);
     if (!launch_fits(tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)tiles);
-    csx::TileArgs a{};
-    a.in = in;
-    a.mask = d_mask;
-    a.out = out;
-    a.n = n;
-    a.offsets = d_offsets;
-    a.counts = d_counts;
-    a.log2c = log2c;
-    const bool aligned = (((uintptr_t)d_in | (uintptr_t)d_mask) & 15) == 0;
+    const bool aligned = (((uintptr_t)wa->in | (uintptr_t)wa->mask) & 15) == 0;
     CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeWriteout, variant == CSX_WRITEOUT_STAGED, bit_order, aligned, a, stream); }),
             "csx_writeout launch");
     return ok();
+}
+
+csx_status csx_writeout(const void* d_in, const uint8_t* d_mask, const uint64_t* d_offsets, const uint32_t* d_counts, void* d_out, uint64_t n,
+                        uint32_t elem_size, uint32_t chunk_len, int bit_order, int variant, hipStream_t stream)
+{
+    csx_writeout_args wa{};
+    wa.in = d_in;
+    wa.mask = d_mask;
+    wa.out = d_out;
+    wa.offsets = d_offsets;
+    wa.offsets_u32 = 0;
+    wa.counts = d_counts;
+    wa.out_base = nullptr;
+    wa.n = n;
+    wa.elem_size = elem_size;
+    wa.chunk_len = chunk_len;
+    wa.bit_order = bit_order;
+    wa.variant = variant;
+    return csx_writeout_ex(&wa, stream);
 }
 
 #define CSX_WRITEOUT_W(W, T)                                                                                                                  \

@@ -56,7 +56,16 @@ hipError_t launch_scan(const uint32_t* counts, uint64_t* offsets, uint64_t m, ui
         hipError_t e = launch_lookback_init(hdr, status, tiles, s);
         if (e != hipSuccess) return e;
     }
-    k_scan<<<(unsigned)tiles, kScanBlock, 0, s>>>(counts, offsets, m, total, hdr, status);
+    k_scan<uint64_t><<<(unsigned)tiles, kScanBlock, 0, s>>>(counts, offsets, m, total, hdr, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan32(const uint32_t* counts, uint32_t* offsets, uint64_t m, uint64_t* total, WsHeader* hdr, uint64_t* status, hipStream_t s)
+{
+    const uint64_t tiles = (m + kScanTile - 1) / kScanTile;
+    hipError_t e = launch_lookback_init(hdr, status, tiles, s);
+    if (e != hipSuccess) return e;
+    k_scan<uint32_t><<<(unsigned)tiles, kScanBlock, 0, s>>>(counts, offsets, m, total, hdr, status);
     return hipGetLastError();
 }
 

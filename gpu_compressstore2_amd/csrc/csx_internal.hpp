@@ -16,6 +16,8 @@ hipError_t launch_lookback_init(WsHeader* hdr, uint64_t* status, uint64_t tiles,
 // initialized: the look-back state was already zeroed (by launch_popc)
 hipError_t launch_scan(const uint32_t* counts, uint64_t* offsets, uint64_t m, uint64_t* total, WsHeader* hdr, uint64_t* status, hipStream_t s,
                        bool initialized = false);
+// 32-bit offsets (counts and offsets may alias)
+hipError_t launch_scan32(const uint32_t* counts, uint32_t* offsets, uint64_t m, uint64_t* total, WsHeader* hdr, uint64_t* status, hipStream_t s);
 hipError_t launch_tree_from_offsets(uint32_t* pss, const uint64_t* ex, uint64_t m, hipStream_t s);
 hipError_t launch_tree_offsets(const uint32_t* tree, uint32_t* out, uint64_t m, hipStream_t s);
 hipError_t launch_store_u64(uint64_t* dst, uint64_t v, hipStream_t s);
@@ -26,25 +28,19 @@ template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool a
 template <int W>
 hipError_t launch_tile_counts(int order, bool aligned, const uint8_t* mask, uint64_t n, uint32_t* tcounts, uint64_t* gsums, uint32_t log2g,
                               hipStream_t s);
-template <int W>
-hipError_t launch_thread_per_chunk(int order, const uint8_t* in, const uint8_t* mask, const uint64_t* offsets, const uint32_t* counts, uint8_t* out,
-                                   uint64_t n, uint32_t log2c, hipStream_t s);
+template <int W> hipError_t launch_thread_per_chunk(int order, const TileArgs& a, hipStream_t s);
 
 extern template hipError_t launch_tile<1>(int, bool, int, bool, const TileArgs&, hipStream_t);
 extern template hipError_t launch_tile<2>(int, bool, int, bool, const TileArgs&, hipStream_t);
 extern template hipError_t launch_tile<4>(int, bool, int, bool, const TileArgs&, hipStream_t);
 extern template hipError_t launch_tile<8>(int, bool, int, bool, const TileArgs&, hipStream_t);
 extern template hipError_t launch_tile_counts<1>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, hipStream_t);
-extern template hipError_t launch_thread_per_chunk<1>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
-                                                      uint32_t, hipStream_t);
+extern template hipError_t launch_thread_per_chunk<1>(int, const TileArgs&, hipStream_t);
 extern template hipError_t launch_tile_counts<2>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, hipStream_t);
-extern template hipError_t launch_thread_per_chunk<2>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
-                                                      uint32_t, hipStream_t);
+extern template hipError_t launch_thread_per_chunk<2>(int, const TileArgs&, hipStream_t);
 extern template hipError_t launch_tile_counts<4>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, hipStream_t);
-extern template hipError_t launch_thread_per_chunk<4>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
-                                                      uint32_t, hipStream_t);
+extern template hipError_t launch_thread_per_chunk<4>(int, const TileArgs&, hipStream_t);
 extern template hipError_t launch_tile_counts<8>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, hipStream_t);
-extern template hipError_t launch_thread_per_chunk<8>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
-                                                      uint32_t, hipStream_t);
+extern template hipError_t launch_thread_per_chunk<8>(int, const TileArgs&, hipStream_t);
 
 } // namespace csx

@@ -139,9 +139,12 @@ constexpr int kScanBlock = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanBlock * kScanItems;
 
-static __global__ __launch_bounds__(kScanBlock) void k_scan(
-    const uint32_t* __restrict__ counts, uint64_t* __restrict__ offsets, uint64_t m, uint64_t* __restrict__ total, WsHeader* hdr,
-    uint64_t* status)
+// OutT = uint64_t, or uint32_t for the reference's 32-bit offsets.  counts
+// and offsets may alias (in-place scan): every element is read by the
+// thread that later writes it, before the look-back.
+template <typename OutT>
+__global__ __launch_bounds__(kScanBlock) void k_scan(const uint32_t* counts, OutT* offsets, uint64_t m, uint64_t* __restrict__ total, WsHeader* hdr,
+                                                     uint64_t* status)
 {
     __shared__ uint64_t s_scr[kScanBlock / kWave];
     __shared__ uint64_t s_tile;
@@ -190,13 +193,13 @@ static __global__ __launch_bounds__(kScanBlock) void k_scan(
     if (base + kScanItems <= m) {
 #pragma unroll
         for (int k = 0; k < kScanItems; k++) {
-            offsets[base + k] = run;
+            offsets[base + k] = (OutT)run;
             run += v[k];
         }
     }
     else {
         for (int k = 0; k < kScanItems; k++) {
-            if (base + k < m) offsets[base + k] = run;
+            if (base + k < m) offsets[base + k] = (OutT)run;
             run += v[k];
         }
     }
@@ -419,10 +422,10 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
             (void)block_exclusive_sum<C::BLOCK>((uint64_t)c, sh.scr, &tot);
             extra = tot;
         }
-        if (t == 0) sh.off = a.offsets[chunk] + extra;
+        if (t == 0) sh.off = (a.offsets ? a.offsets[chunk] : (uint64_t)a.offsets32[chunk]) + extra;
         __syncthreads();
     }
-    const uint64_t toff = sh.off;
+    const uint64_t toff = sh.off + (a.out_base ? *a.out_base : 0ull);
 
     // ---- ranks, then staging (or direct stores) ----
     const uintptr_t gstart = (uintptr_t)(a.out + toff * W);
@@ -576,14 +579,19 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint8_t* __restrict__
 // phase 3, thread-per-chunk variant (src/kernels/kernel_3pass.cuh:578-639)
 // ======================================================================
 template <int W, int ORDER>
-__global__ __launch_bounds__(256) void k_thread_per_chunk(
-    const uint8_t* __restrict__ in, const uint8_t* __restrict__ mask, const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ counts,
-    uint8_t* __restrict__ out, uint64_t n, uint32_t log2c)
+__global__ __launch_bounds__(256) void k_thread_per_chunk(TileArgs a)
 {
+    const uint8_t* __restrict__ in = a.in;
+    const uint8_t* __restrict__ mask = a.mask;
+    const uint32_t* __restrict__ counts = a.counts;
+    uint8_t* __restrict__ out = a.out;
+    const uint64_t n = a.n;
+    const uint32_t log2c = a.log2c;
+    const uint64_t base = a.out_base ? *a.out_base : 0ull;
     const uint64_t chunks = (n + (1ull << log2c) - 1) >> log2c;
     for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < chunks; c += (uint64_t)gridDim.x * blockDim.x) {
         if (counts && counts[c] == 0) continue;
-        uint64_t o = offsets[c];
+        uint64_t o = base + (a.offsets ? a.offsets[c] : (uint64_t)a.offsets32[c]);
         const uint64_t e_begin = c << log2c;
         const uint64_t e_end = min(n, e_begin + (1ull << log2c));
         for (uint64_t i = e_begin; i < e_end; i += 8) {

@@ -68,24 +68,21 @@ hipError_t launch_tile_counts(int order, bool aligned, const uint8_t* mask, uint
     return hipGetLastError();
 }
 
-template <int W>
-hipError_t launch_thread_per_chunk(int order, const uint8_t* in, const uint8_t* mask, const uint64_t* offsets, const uint32_t* counts, uint8_t* out,
-                                   uint64_t n, uint32_t log2c, hipStream_t s)
+template <int W> hipError_t launch_thread_per_chunk(int order, const TileArgs& a, hipStream_t s)
 {
-    const uint64_t chunks = (n + (1ull << log2c) - 1) >> log2c;
+    const uint64_t chunks = (a.n + (1ull << a.log2c) - 1) >> a.log2c;
     uint64_t g = (chunks + 255) / 256;
     if (g < 1) g = 1;
     if (g > 65536) g = 65536;
     if (order == kBitMsb)
-        k_thread_per_chunk<W, kBitMsb><<<(unsigned)g, 256, 0, s>>>(in, mask, offsets, counts, out, n, log2c);
+        k_thread_per_chunk<W, kBitMsb><<<(unsigned)g, 256, 0, s>>>(a);
     else
-        k_thread_per_chunk<W, kBitLsb><<<(unsigned)g, 256, 0, s>>>(in, mask, offsets, counts, out, n, log2c);
+        k_thread_per_chunk<W, kBitLsb><<<(unsigned)g, 256, 0, s>>>(a);
     return hipGetLastError();
 }
 
 template hipError_t launch_tile<CSX_TILE_W>(int, bool, int, bool, const TileArgs&, hipStream_t);
 template hipError_t launch_tile_counts<CSX_TILE_W>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, hipStream_t);
-template hipError_t launch_thread_per_chunk<CSX_TILE_W>(int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*, uint8_t*, uint64_t,
-                                                        uint32_t, hipStream_t);
+template hipError_t launch_thread_per_chunk<CSX_TILE_W>(int, const TileArgs&, hipStream_t);
 
 } // namespace csx

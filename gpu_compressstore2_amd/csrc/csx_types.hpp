@@ -31,9 +31,11 @@ struct TileArgs {
     uint8_t* out;
     uint64_t n;
     // three-phase writeout
-    const uint64_t* offsets;  // exclusive per-chunk offsets
-    const uint32_t* counts;   // nullable: skip tiles whose chunks are empty
+    const uint64_t* offsets;    // exclusive per-chunk offsets (or:)
+    const uint32_t* offsets32;  // the same as 32-bit values, used when offsets is null
+    const uint32_t* counts;     // nullable: skip tiles whose chunks are empty
     uint32_t log2c;
+    const uint64_t* out_base;   // nullable: device value added to every output offset
     // fused
     WsHeader* hdr;
     uint64_t* status;

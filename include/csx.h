@@ -102,6 +102,13 @@ csx_status csx_scan_workspace_size(uint64_t chunks, size_t* bytes);
 csx_status csx_scan(const uint32_t* d_counts, uint64_t* d_offsets, uint64_t chunks, uint64_t* d_total, void* d_ws, size_t ws_bytes,
                     hipStream_t stream);
 
+/* The same scan with the reference's 32-bit offsets (its d_pss2 arrays,
+ * src/benchmarks.cuh:91-93); d_offsets may equal d_counts (in place, as the
+ * CUB ExclusiveSum + copy-back of launch_cub_pss, src/cub_wraps.cuh:52-79).
+ * Offsets wrap modulo 2^32 past 2^32 - 1 selected elements. */
+csx_status csx_scan_u32(const uint32_t* d_counts, uint32_t* d_offsets, uint64_t chunks, uint64_t* d_total, void* d_ws, size_t ws_bytes,
+                        hipStream_t stream);
+
 /* The reference's in-place up-sweep tree (src/kernels/kernel_3pass.cuh:69-153):
  * d_pss[j] becomes the sum of counts (j - lowbit(j+1), j]; *d_total (nullable)
  * the exact sum.  d_pss is read as counts and overwritten.  Workspace as csx_scan. */
@@ -119,6 +126,27 @@ csx_status csx_tree_offsets(const uint32_t* d_tree, uint32_t* d_offsets, uint64_
  * and launch_3pass_proc_none<T> (kernel_3pass.cuh:642-693). */
 csx_status csx_writeout(const void* d_in, const uint8_t* d_mask, const uint64_t* d_offsets, const uint32_t* d_counts, void* d_out, uint64_t n,
                         uint32_t elem_size, uint32_t chunk_len, int bit_order, int variant, hipStream_t stream);
+/* All writeout options in one argument block:
+ *   offsets_u32  the offsets are the reference's uint32_t d_pss values
+ *   out_base     nullable device pointer; its value is added to every output
+ *                index, the `offset` argument of the reference's
+ *                switch_3pass_proc_true_striding (kernel_3pass.cuh:491-504,
+ *                used by its multi-stream split, src/benchmarks.cuh:284-287) */
+typedef struct {
+    const void* in;
+    const uint8_t* mask;
+    void* out;
+    const void* offsets;
+    int offsets_u32;
+    const uint32_t* counts;
+    const uint64_t* out_base;
+    uint64_t n;
+    uint32_t elem_size;
+    uint32_t chunk_len;
+    int bit_order;
+    int variant;
+} csx_writeout_args;
+csx_status csx_writeout_ex(const csx_writeout_args* args, hipStream_t stream);
 csx_status csx_writeout_w1(const uint8_t* d_in, const uint8_t* d_mask, const uint64_t* d_offsets, const uint32_t* d_counts, uint8_t* d_out,
                            uint64_t n, uint32_t chunk_len, int bit_order, int variant, hipStream_t stream);
 csx_status csx_writeout_w2(const uint16_t* d_in, const uint8_t* d_mask, const uint64_t* d_offsets, const uint32_t* d_counts, uint16_t* d_out,

@@ -291,3 +291,33 @@ def test_full_size_2p28_index_identity(p, pipeline):
     assert torch.equal(out[:k], expect)
     del bits, mask, data, out, expect
     torch.cuda.empty_cache()
+
+
+def test_scan_u32_in_place_and_separate(O):
+    rng = np.random.default_rng(9)
+    for m in (1, 4096, 4097, 262144 + 5):
+        counts = rng.integers(0, 1025, m).astype(np.uint32)
+        expect, total = O.exclusive_scan(counts)
+        d = dev(counts)
+        offs, tot = csx.scan_u32(d)
+        assert int(tot.item()) == total and (host(offs, np.uint32) == expect.astype(np.uint32)).all()
+        offs2, tot2 = csx.scan_u32(d, offsets=d)  # in place
+        assert int(tot2.item()) == total and (host(d, np.uint32) == expect.astype(np.uint32)).all()
+
+
+@pytest.mark.parametrize("variant", [csx.WRITEOUT_STAGED, csx.WRITEOUT_DIRECT, csx.WRITEOUT_THREAD_PER_CHUNK])
+def test_writeout_u32_offsets_and_out_base(O, variant):
+    n, chunk = 200003, 512
+    for w in (1, 4, 8):
+        data = O.fill_values(n, NP[w], seed=w + 40)
+        mask = O.mask_bernoulli(n, 0.4, seed=w + 41)
+        counts = O.popc(mask, n, chunk)
+        offs, total = O.exclusive_scan(counts)
+        expect, ke = O.compact(data, mask, n)
+        base = 777
+        d_out = torch.zeros(n + base + 16, dtype=dev(data).dtype, device="cuda")
+        d_base = torch.tensor([base], dtype=torch.int64, device="cuda")
+        csx.writeout(dev(data), dev(mask), dev(offs.astype(np.uint32)), n, chunk, variant=variant, out=d_out, out_base=d_base)
+        got = host(d_out, NP[w])
+        assert (got[base:base + ke] == expect).all()
+        assert (got[:base] == 0).all() and (got[base + ke:] == 0).all()
