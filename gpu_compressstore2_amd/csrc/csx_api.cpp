@@ -112,9 +112,7 @@ ScanPlan plan_scan(uint64_t chunks)
     return p;
 }
 
-// Compaction workspace: header | status[max(fused tiles, scan tiles)] |
-// u32 counts[chunks] | u64 offsets[chunks].  The two pipelines share the
-// header and the status words.
+// Compaction workspace:
 // header | status[max(fused tiles, scan tiles)] | u32 counts[chunks] |
 // u64 offsets[chunks] | u32 tile counts[tiles] | u64 group sums[groups].
 // The pipelines share the header and the status words.
@@ -162,13 +160,13 @@ hipError_t dispatch_tile(uint32_t w, int mode, bool staged, int order, bool alig
 }
 
 hipError_t dispatch_tile_counts(uint32_t w, int order, bool aligned, const uint8_t* mask, uint64_t n, uint32_t* tcounts, uint64_t* gsums,
-                                uint32_t log2g, hipStream_t s)
+                                uint32_t log2g, csx::WsHeader* hdr, hipStream_t s)
 {
     switch (w) {
-        case 1: return csx::launch_tile_counts<1>(order, aligned, mask, n, tcounts, gsums, log2g, s);
-        case 2: return csx::launch_tile_counts<2>(order, aligned, mask, n, tcounts, gsums, log2g, s);
-        case 4: return csx::launch_tile_counts<4>(order, aligned, mask, n, tcounts, gsums, log2g, s);
-        default: return csx::launch_tile_counts<8>(order, aligned, mask, n, tcounts, gsums, log2g, s);
+        case 1: return csx::launch_tile_counts<1>(order, aligned, mask, n, tcounts, gsums, log2g, hdr, s);
+        case 2: return csx::launch_tile_counts<2>(order, aligned, mask, n, tcounts, gsums, log2g, hdr, s);
+        case 4: return csx::launch_tile_counts<4>(order, aligned, mask, n, tcounts, gsums, log2g, hdr, s);
+        default: return csx::launch_tile_counts<8>(order, aligned, mask, n, tcounts, gsums, log2g, hdr, s);
     }
 }
 
@@ -442,7 +440,7 @@ csx_status csx_compact(const void* d_in, const uint8_t* d_mask, void* d_out, uin
     else if (pipeline == CSX_PIPELINE_TWO_PASS) {
         auto tcounts = reinterpret_cast<uint32_t*>(base + p.tcounts_off);
         auto gsums = reinterpret_cast<uint64_t*>(base + p.gsums_off);
-        CSX_HIP(dispatch_tile_counts(elem_size, bit_order, aligned, d_mask, n, tcounts, gsums, p.log2g, stream), "csx_compact tile counts");
+        CSX_HIP(dispatch_tile_counts(elem_size, bit_order, aligned, d_mask, n, tcounts, gsums, p.log2g, hdr, stream), "csx_compact tile counts");
         csx::TileArgs a{};
         a.in = in;
         a.mask = d_mask;

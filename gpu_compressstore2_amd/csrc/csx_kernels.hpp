@@ -524,10 +524,15 @@ __global__ __launch_bounds__(256) void k_tile_loop(TileArgs a)
 // ======================================================================
 constexpr uint32_t kMaxGroupTiles = 8192;
 
+// Also clears the workspace header (ticket and error word) when `hdr` is set.
 template <int W, int ORDER, bool ALIGNED>
 __global__ __launch_bounds__(256) void k_tile_counts(const uint8_t* __restrict__ mask, uint64_t n, uint32_t* __restrict__ tcounts,
-                                                     uint64_t* __restrict__ gsums, uint32_t log2g)
+                                                     uint64_t* __restrict__ gsums, uint32_t log2g, WsHeader* hdr)
 {
+    if (hdr && blockIdx.x == 0 && threadIdx.x == 0) {
+        hdr->ticket = 0;
+        hdr->error = 0;
+    }
     constexpr uint64_t TE = kTileElems<W>;
     constexpr uint32_t VEC_PER_TILE = (uint32_t)(TE / 128);  // 16-byte mask vectors per tile
     constexpr uint32_t LANES_PER_TILE = VEC_PER_TILE < 64 ? VEC_PER_TILE : 64;

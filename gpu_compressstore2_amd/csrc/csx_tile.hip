@@ -48,22 +48,22 @@ template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool a
 
 template <int W>
 hipError_t launch_tile_counts(int order, bool aligned, const uint8_t* mask, uint64_t n, uint32_t* tcounts, uint64_t* gsums, uint32_t log2g,
-                              hipStream_t s)
+                              WsHeader* hdr, hipStream_t s)
 {
     const uint64_t ntiles = (n + kTileElems<W> - 1) / kTileElems<W>;
     const uint64_t groups = (ntiles + (1ull << log2g) - 1) >> log2g;
     if (groups == 0) return hipSuccess;
     if (order == kBitMsb) {
         if (aligned)
-            k_tile_counts<W, kBitMsb, true><<<(unsigned)groups, 256, 0, s>>>(mask, n, tcounts, gsums, log2g);
+            k_tile_counts<W, kBitMsb, true><<<(unsigned)groups, 256, 0, s>>>(mask, n, tcounts, gsums, log2g, hdr);
         else
-            k_tile_counts<W, kBitMsb, false><<<(unsigned)groups, 256, 0, s>>>(mask, n, tcounts, gsums, log2g);
+            k_tile_counts<W, kBitMsb, false><<<(unsigned)groups, 256, 0, s>>>(mask, n, tcounts, gsums, log2g, hdr);
     }
     else {
         if (aligned)
-            k_tile_counts<W, kBitLsb, true><<<(unsigned)groups, 256, 0, s>>>(mask, n, tcounts, gsums, log2g);
+            k_tile_counts<W, kBitLsb, true><<<(unsigned)groups, 256, 0, s>>>(mask, n, tcounts, gsums, log2g, hdr);
         else
-            k_tile_counts<W, kBitLsb, false><<<(unsigned)groups, 256, 0, s>>>(mask, n, tcounts, gsums, log2g);
+            k_tile_counts<W, kBitLsb, false><<<(unsigned)groups, 256, 0, s>>>(mask, n, tcounts, gsums, log2g, hdr);
     }
     return hipGetLastError();
 }
@@ -82,7 +82,7 @@ template <int W> hipError_t launch_thread_per_chunk(int order, const TileArgs& a
 }
 
 template hipError_t launch_tile<CSX_TILE_W>(int, bool, int, bool, const TileArgs&, hipStream_t);
-template hipError_t launch_tile_counts<CSX_TILE_W>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, hipStream_t);
+template hipError_t launch_tile_counts<CSX_TILE_W>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, WsHeader*, hipStream_t);
 template hipError_t launch_thread_per_chunk<CSX_TILE_W>(int, const TileArgs&, hipStream_t);
 
 } // namespace csx
