@@ -4,13 +4,13 @@
 
 Workload: 2^28 four-byte elements (float32 uniform in [0,1), moved as opaque
 32-bit words) with packed MSB-first Bernoulli masks at 1 %, 50 % and 99 %
-selectivity.  A step is one csx_compact call (two-pass pipeline)
+selectivity.  A step is one csx_compact call (three-phase pipeline)
 over one input set, inputs already resident in HBM; three input sets rotate
 so the 256 MB Infinity Cache cannot serve a step from the previous one.
 
 Metric (effective GB/s) = (data bytes read + mask bytes read + selected bytes
 written) / time = (4N + N/8 + 4k) / t.  `value` is the 50 % point; the other
-selectivities, the three-phase and fused pipelines and a plain device copy are reported
+selectivities, the two-pass and fused pipelines and a plain device copy are reported
 under "sweep".  "roofline" prices the dominant kernel (the staged writeout)
 against the 8 TB/s HBM peak with its duration from HIP events recorded on its
 own stream; "cpu_baseline" times the reference's CPU compaction
@@ -86,11 +86,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline")
     ap.add_argument("--sels", default="0.01,0.5,0.99", help="selectivities to sweep (0.5 is always included)")
-    ap.add_argument("--pipelines", default="two_pass,three_phase,fused", help="pipelines to sweep (two_pass is always included)")
+    ap.add_argument("--pipelines", default="three_phase,two_pass,fused", help="pipelines to sweep (three_phase is always included)")
     args = ap.parse_args()
     sels = sorted(set(float(x) for x in args.sels.split(",")) | {0.5})
-    all_pipes = {"two_pass": csx.PIPELINE_TWO_PASS, "three_phase": csx.PIPELINE_THREE_PHASE, "fused": csx.PIPELINE_FUSED}
-    pipes = [(k, all_pipes[k]) for k in all_pipes if k == "two_pass" or k in args.pipelines.split(",")]
+    all_pipes = {"three_phase": csx.PIPELINE_THREE_PHASE, "two_pass": csx.PIPELINE_TWO_PASS, "fused": csx.PIPELINE_FUSED}
+    pipes = [(k, all_pipes[k]) for k in all_pipes if k == "three_phase" or k in args.pipelines.split(",")]
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the compaction shards "
@@ -147,7 +147,7 @@ def main():
                 print(f"[bench] MISMATCH {name} p={p}", file=sys.stderr)
                 sys.exit(1)
             del expect
-            if name == "two_pass" and p == 0.5:
+            if name == "three_phase" and p == 0.5:
                 main_result = (gbs, ms / K, kern_avg, step_bytes)
         del masks
 
@@ -158,7 +158,7 @@ def main():
     gbs, ms_step, kern_avg, step_bytes = main_result
     achieved = step_bytes / (kern_avg * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_two_pass_2p28_u32_p50.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_writeout_2p28_u32_p50.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -169,7 +169,7 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "kernel": "csx::k_tile<4,MSB,aligned,two-pass,staged>",
+        "kernel": "csx::k_tile<4,MSB,aligned,writeout,staged> (phase 3)",
         "algorithmic_bytes_per_launch": int(step_bytes),
     }
 
@@ -191,7 +191,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic: float32 uniform [0,1) moved as u32 words, seeded torch generators, MSB-first Bernoulli masks, 3 rotating input sets",
         "config": {
-            "workload": "single-column compaction of 2^28 4-byte elements, Bernoulli(0.5) mask, two-pass pipeline",
+            "workload": "single-column compaction of 2^28 4-byte elements, Bernoulli(0.5) mask, three-phase pipeline (popc, look-back scan, staged writeout), chunk 1024",
             "n": n,
             "elem_bytes": w,
             "selectivity": 0.5,

@@ -239,7 +239,7 @@ def scan_u32(counts, offsets=None, total=None, workspace=None, stream=None):
     return offsets[:m], total
 
 
-def compact(inp, mask, n=None, chunk_len=1024, bit_order=MSB_FIRST, pipeline=PIPELINE_TWO_PASS, out=None, count=None, workspace=None,
+def compact(inp, mask, n=None, chunk_len=1024, bit_order=MSB_FIRST, pipeline=PIPELINE_THREE_PHASE, out=None, count=None, workspace=None,
             stream=None, host_count=True):
     """Full compaction.  Returns (out, k) with k an int (host_count) or a device tensor."""
     _require_device(inp, mask, out, count)

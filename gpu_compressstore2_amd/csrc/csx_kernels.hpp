@@ -285,16 +285,33 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
     const uint64_t tbase = tile * (uint64_t)C::TE;
     const bool partial = tbase + C::TE > n;
 
-    // ---- zero-chunk skipping (three-phase writeout with counts) ----
-    if constexpr (MODE == kModeWriteout) {
-        if (a.counts) {
-            const uint64_t chunks = (n + (1ull << a.log2c) - 1) >> a.log2c;
-            const uint64_t c0 = tbase >> a.log2c;
-            const uint64_t c1 = min(chunks, ((tbase + C::TE - 1) >> a.log2c) + 1);
-            int any = 0;
-            for (uint64_t c = c0 + t; c < c1; c += C::BLOCK) any |= a.counts[c] != 0;
-            if (!__syncthreads_or(any)) return;
+    // Zero-chunk skipping needs no chunk counts here: data vectors are loaded
+    // only where the mask selects something, and a tile whose mask words are
+    // all zero returns right after its mask scan (no data read, no store).
+
+    // two-pass: this thread's share of the selected count before the tile;
+    // independent loads, issued before the mask load
+    uint64_t psum = 0;
+    if constexpr (MODE == kModeTwoPass) {
+        const uint64_t g = tile >> a.log2g;
+        const uint64_t t0 = g << a.log2g;
+        uint64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint64_t j0 = t; j0 < g; j0 += 8 * C::BLOCK) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const uint64_t j = j0 + (uint64_t)k * C::BLOCK;
+                if (j < g) acc[k] += a.gsums[j];
+            }
         }
+        for (uint64_t j0 = t0 + t; j0 < tile; j0 += 8 * C::BLOCK) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const uint64_t j = j0 + (uint64_t)k * C::BLOCK;
+                if (j < tile) acc[k] += a.tcounts[j];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) psum += acc[k];
     }
 
     // ---- mask words, word prefixes, tile count ----
@@ -316,13 +333,6 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         }
         wsum = popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
     }
-    // two-pass: this thread's share of the selected count before the tile
-    uint64_t psum = 0;
-    if constexpr (MODE == kModeTwoPass) {
-        const uint64_t g = tile >> a.log2g;
-        for (uint64_t j = t; j < g; j += C::BLOCK) psum += a.gsums[j];
-        for (uint64_t j = (g << a.log2g) + t; j < tile; j += C::BLOCK) psum += a.tcounts[j];
-    }
     // one block scan of (psum << 16 | wsum): the low field (a tile holds at
     // most 32768 elements) never carries into the high one
     uint64_t packed_total;
@@ -338,6 +348,14 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
             sh.words[4 * t + k] = w[k];
             sh.wpre[4 * t + k] = run;
             run += popc32(w[k]);
+        }
+    }
+    if constexpr (MODE != kModeFused) {
+        // an empty tile has nothing to read or write (the fused form must
+        // still publish its look-back status)
+        if (tile_count == 0) {
+            if (MODE == kModeTwoPass && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = prefix_total;
+            return;
         }
     }
     __syncthreads();
@@ -522,23 +540,25 @@ __global__ __launch_bounds__(256) void k_tile_loop(TileArgs a)
 // two-pass pipeline, pass 1: selected count of every tile and of every
 // group of 2^log2g consecutive tiles (one workgroup per group)
 // ======================================================================
-constexpr uint32_t kMaxGroupTiles = 8192;
+constexpr uint32_t kMaxGroupTiles = 2048;
 
 // Also clears the workspace header (ticket and error word) when `hdr` is set.
 template <int W, int ORDER, bool ALIGNED>
 __global__ __launch_bounds__(256) void k_tile_counts(const uint8_t* __restrict__ mask, uint64_t n, uint32_t* __restrict__ tcounts,
                                                      uint64_t* __restrict__ gsums, uint32_t log2g, WsHeader* hdr)
 {
+    constexpr uint64_t TE = kTileElems<W>;
+    constexpr uint32_t VEC_PER_TILE = (uint32_t)(TE / 128);        // 16-byte mask vectors per tile
+    constexpr uint32_t LANES_PER_TILE = VEC_PER_TILE < 64 ? VEC_PER_TILE : 64;
+    constexpr uint32_t ROUND_TILES = 16;                            // tiles per unrolled round
+    constexpr uint32_t VPL = ROUND_TILES * VEC_PER_TILE / 256;      // vectors per thread and round
     if (hdr && blockIdx.x == 0 && threadIdx.x == 0) {
         hdr->ticket = 0;
         hdr->error = 0;
     }
-    constexpr uint64_t TE = kTileElems<W>;
-    constexpr uint32_t VEC_PER_TILE = (uint32_t)(TE / 128);  // 16-byte mask vectors per tile
-    constexpr uint32_t LANES_PER_TILE = VEC_PER_TILE < 64 ? VEC_PER_TILE : 64;
     __shared__ uint32_t tc[kMaxGroupTiles];
     __shared__ uint64_t scr[4];
-    const uint32_t t = threadIdx.x, lane = lane_id(), wave = t / kWave;
+    const uint32_t t = threadIdx.x, lane = lane_id();
     const uint64_t nbytes = (n + 7) / 8;
     const uint64_t nvec = (nbytes + 15) / 16;
     const uint64_t ntiles = (n + TE - 1) / TE;
@@ -547,14 +567,21 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint8_t* __restrict__
     const uint64_t tile1 = min(ntiles, tile0 + G);
     for (uint64_t j = t; j < G; j += 256) tc[j] = 0;
     __syncthreads();
-    const uint64_t v0 = tile0 * VEC_PER_TILE;
-    const uint64_t v1 = min(nvec, tile1 * VEC_PER_TILE);
-    for (uint64_t vb = v0 + wave * kWave; vb < v1; vb += 256) {
-        const uint64_t v = vb + lane;
-        uint32_t c = 0;
-        if (v < v1) {
-            u32x4 x = load_mask16<ALIGNED>(mask, v * 16, nbytes);
-            uint32_t w[4] = {to_lsb_word<ORDER>(x.x), to_lsb_word<ORDER>(x.y), to_lsb_word<ORDER>(x.z), to_lsb_word<ORDER>(x.w)};
+    const uint64_t v_end = min(nvec, tile1 * VEC_PER_TILE);
+    for (uint64_t r0 = tile0; r0 < tile1; r0 += ROUND_TILES) {
+        const uint64_t vr = r0 * VEC_PER_TILE;  // first vector of the round
+        uint32_t c[VPL];
+        // issue every load of the round first
+        u32x4 x[VPL];
+#pragma unroll
+        for (uint32_t j = 0; j < VPL; j++) {
+            const uint64_t v = vr + j * 256 + t;
+            x[j] = v < v_end ? load_mask16<ALIGNED>(mask, v * 16, nbytes) : u32x4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < VPL; j++) {
+            const uint64_t v = vr + j * 256 + t;
+            uint32_t w[4] = {to_lsb_word<ORDER>(x[j].x), to_lsb_word<ORDER>(x[j].y), to_lsb_word<ORDER>(x[j].z), to_lsb_word<ORDER>(x[j].w)};
             const uint64_t e0 = v * 128;
             if (e0 + 128 > n) {
 #pragma unroll
@@ -563,11 +590,16 @@ __global__ __launch_bounds__(256) void k_tile_counts(const uint8_t* __restrict__
                     w[k] = b >= n ? 0u : low_bits(w[k], (uint32_t)(n - b));
                 }
             }
-            c = popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
+            c[j] = popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
         }
 #pragma unroll
-        for (uint32_t d = LANES_PER_TILE / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
-        if ((lane & (LANES_PER_TILE - 1)) == 0 && v < v1 && c) atomicAdd(&tc[(v - v0) / VEC_PER_TILE], c);
+        for (uint32_t j = 0; j < VPL; j++) {
+            uint32_t cj = c[j];
+#pragma unroll
+            for (uint32_t d = LANES_PER_TILE / 2; d >= 1; d >>= 1) cj += __shfl_xor(cj, d, kWave);
+            const uint64_t v = vr + j * 256 + t;
+            if ((lane & (LANES_PER_TILE - 1)) == 0 && v < v_end && cj) atomicAdd(&tc[(v - tile0 * VEC_PER_TILE) / VEC_PER_TILE], cj);
+        }
     }
     __syncthreads();
     uint64_t s = 0;

@@ -48,11 +48,11 @@ struct TileArgs {
 
 // Tiles per group of the two-pass pipeline: a power of two >= 16 keeping
 // the group count at most 2048 (the writeout sums at most 8 group sums per
-// thread), capped at 8192 tiles per group.
+// thread), capped at 2048 tiles per group (the pass-1 LDS table).
 inline uint32_t two_pass_log2g(uint64_t ntiles)
 {
     uint32_t l = 4;
-    while (l < 13 && ((ntiles + (1ull << l) - 1) >> l) > 2048) l++;
+    while (l < 11 && ((ntiles + (1ull << l) - 1) >> l) > 2048) l++;
     return l;
 }
 

@@ -69,9 +69,10 @@ typedef enum {
 /* whole-pipeline forms of csx_compact */
 typedef enum {
     CSX_PIPELINE_FUSED = 0,       /* one pass: mask popcount, decoupled look-back and staged writeout per tile */
-    CSX_PIPELINE_THREE_PHASE = 1, /* csx_popc -> csx_scan -> csx_writeout (STAGED), the reference's bench10 shape */
+    CSX_PIPELINE_THREE_PHASE = 1, /* csx_popc -> csx_scan -> csx_writeout (STAGED), the reference's bench10 shape;
+                                     the fastest form at low and medium selectivity (default of the bindings) */
     CSX_PIPELINE_TWO_PASS = 2     /* pass 1 counts every tile and group of tiles from the mask; pass 2 is the staged
-                                     writeout, each tile reducing its own offset from those counts (default) */
+                                     writeout, each tile reducing its own offset from those counts */
 } csx_pipeline;
 
 /* Largest element count accepted (offsets are 62-bit in the look-back status words). */
@@ -162,8 +163,7 @@ csx_status csx_writeout_w8(const uint64_t* d_in, const uint8_t* d_mask, const ui
  * *h_count after synchronising the stream.  d_out must hold k elements
  * (n is always enough); nothing at or past d_out[k] is written.
  * chunk_len only matters for CSX_PIPELINE_THREE_PHASE.
- * CSX_PIPELINE_TWO_PASS is the fastest form on MI355X and the default of the
- * language bindings.
+ * CSX_PIPELINE_THREE_PHASE is the default of the language bindings.
  * Replaces the benchN compositions (src/benchmarks.cuh:153-440) -- bench10
  * is CSX_PIPELINE_THREE_PHASE with the staged writeout. */
 csx_status csx_workspace_size(uint64_t n, uint32_t elem_size, uint32_t chunk_len, size_t* bytes);
