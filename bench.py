@@ -151,8 +151,8 @@ def main():
                 main_result = (gbs, ms / K, kern_avg, step_bytes)
         del masks
 
-    # plain device-to-device copy of the input size: what this HBM sustains
-    copy_ms = time_steps(lambda i: out.copy_(datas[i % ROTATE]), K, W)
+    # plain device copy of the input size (csx_copy): what this HBM sustains
+    copy_ms = time_steps(lambda i: csx.copy(out, datas[i % ROTATE]), K, W)
     sweep["copy_4N"] = {"GB/s": round(K * 2 * 4 * n / (copy_ms * 1e-3) / 1e9, 1), "ms_per_step": round(copy_ms / K, 5)}
 
     gbs, ms_step, kern_avg, step_bytes = main_result

@@ -86,6 +86,7 @@ SIGNATURES = {
     "csx_validate": (_int, [_vp, _vp, _u64, _u32, _vp, _vp]),
     "csx_scan_u32": (_int, [_vp, _vp, _u64, _vp, _vp, _sz, _vp]),
     "csx_writeout_ex": (_int, [_vp, _vp]),
+    "csx_copy": (_int, [_vp, _vp, _u64, _vp]),
     "csx_profile_begin": (_int, []),
     "csx_profile_end": (_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
 }
@@ -269,6 +270,14 @@ def validate(expected, actual, length, failures=None, stream=None):
         failures = torch.zeros(1, dtype=torch.int64, device=expected.device)
     _check(lib().csx_validate(_ptr(expected), _ptr(actual), length, expected.element_size(), _ptr(failures), _stream(stream)))
     return failures
+
+
+def copy(dst, src, nbytes=None, stream=None):
+    """Plain device copy (the bandwidth and counter-calibration reference)."""
+    _require_device(dst, src)
+    nbytes = src.numel() * src.element_size() if nbytes is None else nbytes
+    _check(lib().csx_copy(_ptr(dst), _ptr(src), nbytes, _stream(stream)))
+    return dst
 
 
 def profile_begin():

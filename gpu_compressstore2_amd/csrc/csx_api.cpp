@@ -504,6 +504,15 @@ csx_status csx_validate(const void* d_expected, const void* d_actual, uint64_t l
     return ok();
 }
 
+csx_status csx_copy(void* d_dst, const void* d_src, uint64_t bytes, hipStream_t stream)
+{
+    if (bytes == 0) return ok();
+    if (!d_dst || !d_src) return fail(CSX_ERR_INVALID, "null pointer");
+    if ((((uintptr_t)d_dst) | ((uintptr_t)d_src)) & 15) return fail(CSX_ERR_INVALID, "csx_copy needs 16-byte aligned pointers");
+    CSX_HIP(timed(stream, [&] { return csx::launch_copy(d_dst, d_src, bytes, stream); }), "csx_copy launch");
+    return ok();
+}
+
 csx_status csx_profile_begin(void)
 {
     for (auto& p : g_prof.events) {

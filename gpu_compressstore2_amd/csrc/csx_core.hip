@@ -104,3 +104,26 @@ hipError_t launch_validate(const void* a, const void* b, uint64_t len, uint32_t 
 }
 
 } // namespace csx
+
+namespace csx {
+
+// Plain device copy, 16 bytes per lane, grid-stride: the calibration kernel
+// for traffic counters and the bandwidth reference of the benchmarks.
+static __global__ __launch_bounds__(256) void k_copy(const u32x4* __restrict__ src, u32x4* __restrict__ dst, uint64_t nv, const uint8_t* __restrict__ src_tail,
+                                                     uint8_t* __restrict__ dst_tail, uint32_t tail)
+{
+    const uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    for (uint64_t i = tid; i < nv; i += (uint64_t)gridDim.x * blockDim.x) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+    if (tid < tail) dst_tail[tid] = src_tail[tid];
+}
+
+hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s)
+{
+    const uint64_t nv = bytes / 16;
+    const uint32_t tail = (uint32_t)(bytes % 16);
+    k_copy<<<grid_for(nv, 256, 1024), 256, 0, s>>>(static_cast<const u32x4*>(src), static_cast<u32x4*>(dst), nv,
+                                                  static_cast<const uint8_t*>(src) + nv * 16, static_cast<uint8_t*>(dst) + nv * 16, tail);
+    return hipGetLastError();
+}
+
+} // namespace csx

@@ -21,6 +21,8 @@ hipError_t launch_scan32(const uint32_t* counts, uint32_t* offsets, uint64_t m, 
 hipError_t launch_tree_from_offsets(uint32_t* pss, const uint64_t* ex, uint64_t m, hipStream_t s);
 hipError_t launch_tree_offsets(const uint32_t* tree, uint32_t* out, uint64_t m, hipStream_t s);
 hipError_t launch_store_u64(uint64_t* dst, uint64_t v, hipStream_t s);
+// both pointers 16-byte aligned
+hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);
 hipError_t launch_validate(const void* a, const void* b, uint64_t len, uint32_t w, uint64_t* failures, hipStream_t s);
 
 // tile engine, instantiated once per element width (csx_tile.hip)

@@ -190,6 +190,12 @@ csx_status csx_workspace_error(const void* d_ws, uint32_t* h_error, hipStream_t 
 csx_status csx_validate(const void* d_expected, const void* d_actual, uint64_t len, uint32_t elem_size, uint64_t* d_failures,
                         hipStream_t stream);
 
+/* Plain device copy of `bytes` bytes (16-byte aligned pointers): the
+ * bandwidth reference the benchmark drivers report beside the compaction and
+ * the known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters.
+ * No counterpart in the reference. */
+csx_status csx_copy(void* d_dst, const void* d_src, uint64_t bytes, hipStream_t stream);
+
 /* ---------------------------------------------------------------- profiling
  * Kernel timing for benchmark drivers: after csx_profile_begin(), every
  * csx_compact / csx_writeout call on this host thread records a HIP event
