@@ -87,6 +87,15 @@ SIGNATURES = {
     "csx_scan_u32": (_int, [_vp, _vp, _u64, _vp, _vp, _sz, _vp]),
     "csx_writeout_ex": (_int, [_vp, _vp]),
     "csx_copy": (_int, [_vp, _vp, _u64, _vp]),
+    "csx_gen_mask_bernoulli": (_int, [_vp, _u64, ctypes.c_double, _u64, _int, _vp]),
+    "csx_gen_mask_pattern": (_int, [_vp, _u64, _u32, _u32, _int, _vp]),
+    "csx_zipf_table": (_int, [ctypes.c_double, _u32, _vp, _vp]),
+    "csx_gen_mask_zipf_blocks": (_int, [_vp, _u64, _u32, ctypes.c_double, _u32, _u64, _int, _vp]),
+    "csx_gen_runs_workspace_size": (_int, [_u64, ctypes.POINTER(_sz)]),
+    "csx_gen_mask_runs": (_int, [_vp, _u64, _u32, _u64, _int, _vp, _sz, _vp]),
+    "csx_gen_values": (_int, [_vp, _u64, _u32, _int, _u64, _vp]),
+    "csx_pattern_count": (_int, [_u64, _u32, _u32, ctypes.POINTER(_u64)]),
+    "csx_pattern_compact": (_int, [_vp, _vp, _u64, _u32, _u32, _u32, _vp, ctypes.POINTER(_u64), _vp]),
     "csx_profile_begin": (_int, []),
     "csx_profile_end": (_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
 }
@@ -270,6 +279,85 @@ def validate(expected, actual, length, failures=None, stream=None):
         failures = torch.zeros(1, dtype=torch.int64, device=expected.device)
     _check(lib().csx_validate(_ptr(expected), _ptr(actual), length, expected.element_size(), _ptr(failures), _stream(stream)))
     return failures
+
+
+def pattern_count(n, pattern, pattern_length):
+    k = _u64(0)
+    _check(lib().csx_pattern_count(n, pattern, pattern_length, ctypes.byref(k)))
+    return k.value
+
+
+def pattern_compact(inp, pattern, pattern_length, n=None, out=None, count=None, stream=None):
+    """Compaction by a periodic MSB-first pattern of pattern_length bits (no mask)."""
+    _require_device(inp, out, count)
+    n = inp.numel() if n is None else n
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=inp.dtype, device=inp.device)
+    k = _u64(0)
+    _check(lib().csx_pattern_compact(_ptr(inp), _ptr(out), n, inp.element_size(), pattern, pattern_length, _ptr(count), ctypes.byref(k),
+                                     _stream(stream)))
+    return out, k.value
+
+
+VALUES_INTEGER = 0
+VALUES_FLOAT = 1
+
+
+def _mask_buffer(n, out, device):
+    if out is None:
+        out = torch.empty(max((n + 7) // 8, 1), dtype=torch.uint8, device=device or "cuda")
+    _require_device(out)
+    return out
+
+
+def gen_mask_bernoulli(n, p, seed=42, bit_order=MSB_FIRST, out=None, device=None, stream=None):
+    """Seeded Bernoulli mask on the device (seed 42, MSB-first = the reference's uniform mask)."""
+    out = _mask_buffer(n, out, device)
+    _check(lib().csx_gen_mask_bernoulli(_ptr(out), n, float(p), seed, bit_order, _stream(stream)))
+    return out
+
+
+def gen_mask_pattern(n, pattern, pattern_length, bit_order=MSB_FIRST, out=None, device=None, stream=None):
+    out = _mask_buffer(n, out, device)
+    _check(lib().csx_gen_mask_pattern(_ptr(out), n, pattern, pattern_length, bit_order, _stream(stream)))
+    return out
+
+
+def zipf_table(s, levels):
+    import numpy as np
+
+    cdf = np.zeros(levels, np.uint32)
+    thr = np.zeros(levels, np.uint64)
+    _check(lib().csx_zipf_table(float(s), levels, ctypes.c_void_p(cdf.ctypes.data), ctypes.c_void_p(thr.ctypes.data)))
+    return cdf, thr
+
+
+def gen_mask_zipf_blocks(n, block_len=4096, s=1.2, levels=65, seed=1, bit_order=MSB_FIRST, out=None, device=None, stream=None):
+    out = _mask_buffer(n, out, device)
+    _check(lib().csx_gen_mask_zipf_blocks(_ptr(out), n, block_len, float(s), levels, seed, bit_order, _stream(stream)))
+    return out
+
+
+def gen_mask_runs(n, mean_run=256, seed=1, bit_order=MSB_FIRST, out=None, device=None, workspace=None, stream=None):
+    out = _mask_buffer(n, out, device)
+    need = _sz(0)
+    _check(lib().csx_gen_runs_workspace_size(n, ctypes.byref(need)))
+    if workspace is None:
+        workspace = alloc_workspace(need.value, out.device)
+    _check(lib().csx_gen_mask_runs(_ptr(out), n, mean_run, seed, bit_order, _ptr(workspace), workspace.numel(), _stream(stream)))
+    return out
+
+
+_VALUE_DTYPES = {(1, 0): torch.uint8, (2, 0): torch.int16, (4, 0): torch.int32, (8, 0): torch.int64, (4, 1): torch.float32, (8, 1): torch.float64}
+
+
+def gen_values(n, elem_size, kind=VALUES_INTEGER, seed=1, out=None, device=None, stream=None):
+    """Seeded values on the device (integers, or floats in [0, 1))."""
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=_VALUE_DTYPES[(elem_size, kind)], device=device or "cuda")
+    _require_device(out)
+    _check(lib().csx_gen_values(_ptr(out), n, elem_size, kind, seed, _stream(stream)))
+    return out
 
 
 def copy(dst, src, nbytes=None, stream=None):

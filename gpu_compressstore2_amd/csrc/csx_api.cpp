@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <string>
@@ -501,6 +502,148 @@ csx_status csx_validate(const void* d_expected, const void* d_actual, uint64_t l
     if (len == 0) return ok();
     if (!d_expected || !d_actual || !d_failures) return fail(CSX_ERR_INVALID, "null pointer");
     CSX_HIP(csx::launch_validate(d_expected, d_actual, len, elem_size, d_failures, stream), "csx_validate launch");
+    return ok();
+}
+
+csx_status csx_pattern_count(uint64_t n, uint32_t pattern, uint32_t pattern_length, uint64_t* count)
+{
+    if (pattern_length < 1 || pattern_length > 32) return fail(CSX_ERR_INVALID, "pattern length %u must be in [1, 32]", pattern_length);
+    if (!count) return fail(CSX_ERR_INVALID, "null count pointer");
+    uint64_t per = 0, head = 0;
+    const uint64_t rem = n % pattern_length;
+    for (uint32_t p = 0; p < pattern_length; p++) {
+        if ((pattern >> (31 - p)) & 1u) {
+            per++;
+            if (p < rem) head++;
+        }
+    }
+    *count = (n / pattern_length) * per + head;
+    return ok();
+}
+
+csx_status csx_pattern_compact(const void* d_in, void* d_out, uint64_t n, uint32_t elem_size, uint32_t pattern, uint32_t pattern_length,
+                               uint64_t* d_count, uint64_t* h_count, hipStream_t stream)
+{
+    csx_status s;
+    uint64_t k = 0;
+    if ((s = check_width(elem_size)) || (s = check_n(n)) || (s = csx_pattern_count(n, pattern, pattern_length, &k))) return s;
+    if (k > 0 && (!d_in || !d_out)) return fail(CSX_ERR_INVALID, "null input or output pointer");
+    if (((uintptr_t)d_in % elem_size) || ((uintptr_t)d_out % elem_size))
+        return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
+    CSX_HIP(timed(stream, [&] { return csx::launch_pattern(elem_size, d_in, d_out, k, pattern, pattern_length, stream); }), "csx_pattern_compact");
+    if (d_count) CSX_HIP(csx::launch_store_u64(d_count, k, stream), "csx_pattern_compact count");
+    if (h_count) *h_count = k;
+    return ok();
+}
+
+// ---------------------------------------------------------------- generators
+
+static uint64_t bernoulli_threshold(double p) { return (uint64_t)(p * UINT32_MAX); }
+
+csx_status csx_gen_mask_bernoulli(uint8_t* d_mask, uint64_t n, double p, uint64_t seed, int bit_order, hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_order(bit_order)) || (s = check_n(n))) return s;
+    if (!(p >= 0.0 && p <= 1.0)) return fail(CSX_ERR_INVALID, "selectivity must be in [0, 1]");
+    if (n == 0) return ok();
+    if (!d_mask) return fail(CSX_ERR_INVALID, "null mask pointer");
+    CSX_HIP(csx::launch_gen_bernoulli(d_mask, n, bernoulli_threshold(p), seed, bit_order, stream), "csx_gen_mask_bernoulli");
+    return ok();
+}
+
+csx_status csx_gen_mask_pattern(uint8_t* d_mask, uint64_t n, uint32_t pattern, uint32_t pattern_length, int bit_order, hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_order(bit_order)) || (s = check_n(n))) return s;
+    if (pattern_length < 1 || pattern_length > 32) return fail(CSX_ERR_INVALID, "pattern length %u must be in [1, 32]", pattern_length);
+    if (n == 0) return ok();
+    if (!d_mask) return fail(CSX_ERR_INVALID, "null mask pointer");
+    CSX_HIP(csx::launch_gen_pattern(d_mask, n, pattern, pattern_length, bit_order, stream), "csx_gen_mask_pattern");
+    return ok();
+}
+
+csx_status csx_zipf_table(double s_exp, uint32_t levels, uint32_t* cdf, uint64_t* thr)
+{
+    if (levels < 2 || levels > 128) return fail(CSX_ERR_INVALID, "zipf levels %u must be in [2, 128]", levels);
+    if (!cdf || !thr) return fail(CSX_ERR_INVALID, "null table pointer");
+    double norm = 0.0;
+    for (uint32_t r = 1; r <= levels; r++) norm += std::pow((double)r, -s_exp);
+    double acc = 0.0;
+    for (uint32_t r = 1; r <= levels; r++) {
+        acc += std::pow((double)r, -s_exp) / norm;
+        const double t = acc * 4294967296.0;
+        cdf[r - 1] = (r == levels || t >= 4294967295.0) ? UINT32_MAX : (uint32_t)t;
+        thr[r - 1] = bernoulli_threshold((double)(r - 1) / (double)(levels - 1));
+    }
+    return ok();
+}
+
+csx_status csx_gen_mask_zipf_blocks(uint8_t* d_mask, uint64_t n, uint32_t block_len, double s_exp, uint32_t levels, uint64_t seed, int bit_order,
+                                    hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_order(bit_order)) || (s = check_n(n))) return s;
+    if (block_len == 0 || block_len % 32) return fail(CSX_ERR_INVALID, "block length %u must be a positive multiple of 32", block_len);
+    uint32_t cdf[128];
+    uint64_t thr[128];
+    if ((s = csx_zipf_table(s_exp, levels, cdf, thr))) return s;
+    if (n == 0) return ok();
+    if (!d_mask) return fail(CSX_ERR_INVALID, "null mask pointer");
+    CSX_HIP(csx::launch_gen_zipf(d_mask, n, block_len, seed, cdf, thr, levels, bit_order, stream), "csx_gen_mask_zipf_blocks");
+    return ok();
+}
+
+struct RunsPlan {
+    uint64_t words;
+    size_t counts_off, before_off, status_off, total;
+};
+
+static RunsPlan plan_runs(uint64_t n)
+{
+    RunsPlan p;
+    p.words = (n + 31) / 32;
+    p.counts_off = sizeof(csx::WsHeader);
+    p.before_off = align_up(p.counts_off + p.words * 4);
+    p.status_off = align_up(p.before_off + p.words * 8);
+    p.total = align_up(p.status_off + csx::scan_tiles(p.words) * 8);
+    return p;
+}
+
+csx_status csx_gen_runs_workspace_size(uint64_t n, size_t* bytes)
+{
+    if (!bytes) return fail(CSX_ERR_INVALID, "null size pointer");
+    *bytes = plan_runs(n).total;
+    return ok();
+}
+
+csx_status csx_gen_mask_runs(uint8_t* d_mask, uint64_t n, uint32_t mean_run, uint64_t seed, int bit_order, void* d_ws, size_t ws_bytes,
+                             hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_order(bit_order)) || (s = check_n(n))) return s;
+    if (mean_run < 2) return fail(CSX_ERR_INVALID, "mean run length must be >= 2");
+    if (n == 0) return ok();
+    if (!d_mask) return fail(CSX_ERR_INVALID, "null mask pointer");
+    const RunsPlan p = plan_runs(n);
+    if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
+    auto base = static_cast<uint8_t*>(d_ws);
+    const uint32_t thr = (uint32_t)((1ull << 32) / mean_run);
+    CSX_HIP(csx::launch_gen_runs(d_mask, n, thr, seed, bit_order, reinterpret_cast<uint32_t*>(base + p.counts_off),
+                                 reinterpret_cast<uint64_t*>(base + p.before_off), reinterpret_cast<csx::WsHeader*>(base),
+                                 reinterpret_cast<uint64_t*>(base + p.status_off), stream),
+            "csx_gen_mask_runs");
+    return ok();
+}
+
+csx_status csx_gen_values(void* d_out, uint64_t n, uint32_t elem_size, int kind, uint64_t seed, hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_width(elem_size)) || (s = check_n(n))) return s;
+    if (kind != CSX_VALUES_INTEGER && kind != CSX_VALUES_FLOAT) return fail(CSX_ERR_INVALID, "unknown value kind %d", kind);
+    if (kind == CSX_VALUES_FLOAT && elem_size < 4) return fail(CSX_ERR_INVALID, "float values need 4- or 8-byte elements");
+    if (n == 0) return ok();
+    if (!d_out || ((uintptr_t)d_out % elem_size)) return fail(CSX_ERR_INVALID, "null or misaligned output pointer");
+    CSX_HIP(csx::launch_gen_values(d_out, n, elem_size, kind, seed, stream), "csx_gen_values");
     return ok();
 }
 

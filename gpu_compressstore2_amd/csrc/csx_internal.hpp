@@ -23,6 +23,16 @@ hipError_t launch_tree_offsets(const uint32_t* tree, uint32_t* out, uint64_t m, 
 hipError_t launch_store_u64(uint64_t* dst, uint64_t v, hipStream_t s);
 // both pointers 16-byte aligned
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);
+// device generators (csx_gen.hip)
+hipError_t launch_gen_bernoulli(uint8_t* mask, uint64_t n, uint64_t thr, uint64_t seed, int order, hipStream_t s);
+hipError_t launch_gen_pattern(uint8_t* mask, uint64_t n, uint32_t pattern, uint32_t L, int order, hipStream_t s);
+hipError_t launch_gen_zipf(uint8_t* mask, uint64_t n, uint32_t block_len, uint64_t seed, const uint32_t* cdf, const uint64_t* thr, uint32_t levels,
+                           int order, hipStream_t s);
+hipError_t launch_gen_runs(uint8_t* mask, uint64_t n, uint32_t thr, uint64_t seed, int order, uint32_t* counts, uint64_t* before, WsHeader* hdr,
+                           uint64_t* status, hipStream_t s);
+hipError_t launch_gen_values(void* out, uint64_t n, uint32_t w, int kind, uint64_t seed, hipStream_t s);
+// periodic-pattern compaction (csx_pattern.hip); k = selected count
+hipError_t launch_pattern(uint32_t w, const void* in, void* out, uint64_t k, uint32_t pattern, uint32_t L, hipStream_t s);
 hipError_t launch_validate(const void* a, const void* b, uint64_t len, uint32_t w, uint64_t* failures, hipStream_t s);
 
 // tile engine, instantiated once per element width (csx_tile.hip)

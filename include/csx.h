@@ -190,6 +190,51 @@ csx_status csx_workspace_error(const void* d_ws, uint32_t* h_error, hipStream_t 
 csx_status csx_validate(const void* d_expected, const void* d_actual, uint64_t len, uint32_t elem_size, uint64_t* d_failures,
                         hipStream_t stream);
 
+/* ---------------------------------------------------------------- pattern
+ * Compaction by a periodic pattern, with no mask in memory: element i is
+ * selected iff bit 31-(i % pattern_length) of `pattern` is set (the pattern
+ * starts at its most significant bit, src/data_generator.cuh:113-128); bits
+ * below the top pattern_length are ignored.  pattern_length in [1, 32].
+ * The count has a closed form, so *h_count (nullable) is set without a
+ * synchronisation; *d_count (nullable) is written on the stream.
+ * Replaces launch_pattern_proc<T> (src/kernels/kernel_pattern.cuh:71-131). */
+csx_status csx_pattern_count(uint64_t n, uint32_t pattern, uint32_t pattern_length, uint64_t* count);
+csx_status csx_pattern_compact(const void* d_in, void* d_out, uint64_t n, uint32_t elem_size, uint32_t pattern, uint32_t pattern_length,
+                               uint64_t* d_count, uint64_t* h_count, hipStream_t stream);
+
+/* ---------------------------------------------------------------- generators
+ * Device-side seeded inputs.  All draw from the reference's generator
+ * (PCG32 XSH-RR, increment 1, seeded by one splitmix64 step,
+ * src/fast_prng.cuh:7-36) with element k at fixed draw positions, so the
+ * device output equals a sequential CPU generator's bit for bit:
+ *   bernoulli: element k selected iff draw k < (uint64)(p * UINT32_MAX) --
+ *              with seed 42 and MSB-first this is generate_mask_uniform
+ *              (src/data_generator.cuh:14-30) for n a multiple of 8;
+ *   pattern:   element k is bit 31-(k % pattern_length) of pattern
+ *              (generate_mask_pattern, src/data_generator.cuh:113-128);
+ *   zipf blocks: block b of block_len elements (a multiple of 32) uses draw
+ *              b*(block_len+1) to pick rank r in [0, levels) with
+ *              P(r) ~ (r+1)^-s and selectivity r/(levels-1), then one draw
+ *              per element;
+ *   runs:      element k flips the running bit iff draw k < 2^32/mean_run
+ *              (alternating runs with geometric lengths of mean mean_run);
+ *   values:    element k <- draw k (1/2/4-byte, low bits) or draws 2k, 2k+1
+ *              (8-byte, high word first); CSX_VALUES_FLOAT gives float32
+ *              (draw>>8)*2^-24 or float64 (u64>>11)*2^-53 in [0, 1).
+ * Bits at index >= n are zero; bytes past ceil(n/8) are not written. */
+#define CSX_VALUES_INTEGER 0
+#define CSX_VALUES_FLOAT 1
+csx_status csx_gen_mask_bernoulli(uint8_t* d_mask, uint64_t n, double p, uint64_t seed, int bit_order, hipStream_t stream);
+csx_status csx_gen_mask_pattern(uint8_t* d_mask, uint64_t n, uint32_t pattern, uint32_t pattern_length, int bit_order, hipStream_t stream);
+/* host-side rank table of the zipf-block generator (levels in [2, 128]) */
+csx_status csx_zipf_table(double s, uint32_t levels, uint32_t* cdf, uint64_t* thresholds);
+csx_status csx_gen_mask_zipf_blocks(uint8_t* d_mask, uint64_t n, uint32_t block_len, double s, uint32_t levels, uint64_t seed, int bit_order,
+                                    hipStream_t stream);
+csx_status csx_gen_runs_workspace_size(uint64_t n, size_t* bytes);
+csx_status csx_gen_mask_runs(uint8_t* d_mask, uint64_t n, uint32_t mean_run, uint64_t seed, int bit_order, void* d_ws, size_t ws_bytes,
+                             hipStream_t stream);
+csx_status csx_gen_values(void* d_out, uint64_t n, uint32_t elem_size, int kind, uint64_t seed, hipStream_t stream);
+
 /* Plain device copy of `bytes` bytes (16-byte aligned pointers): the
  * bandwidth reference the benchmark drivers report beside the compaction and
  * the known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters.

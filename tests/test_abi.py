@@ -108,3 +108,19 @@ def test_error_raises_python_exception():
     with pytest.raises(csx.CsxError) as e:
         csx.workspace_size(10, 3, 1024)
     assert e.value.status == 1
+
+
+def test_zipf_table_matches_oracle(O):
+    for s, levels in ((1.2, 65), (2.0, 17), (0.5, 128), (1.0, 2)):
+        cdf, thr = csx.zipf_table(s, levels)
+        assert (cdf == O.zipf_thresholds(s, levels)).all()
+        assert thr[0] == 0 and thr[-1] == 0xFFFFFFFF
+
+
+def test_pattern_count_matches_oracle(O):
+    for pat, L in ((0xB5000000, 8), (0xDEADBEEF, 32), (0x6C000000, 7), (0x0, 5)):
+        for n in (0, 1, 7, 8, 1000, 12345):
+            mask, _ = O.mask_pattern((n + 7) // 8, pat & ((0xFFFFFFFF << (32 - L)) & 0xFFFFFFFF), L)
+            assert csx.pattern_count(n, pat, L) == O.mask_count(mask, n)
+    with pytest.raises(csx.CsxError):
+        csx.pattern_count(10, 1, 33)

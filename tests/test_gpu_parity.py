@@ -321,3 +321,33 @@ def test_writeout_u32_offsets_and_out_base(O, variant):
         got = host(d_out, NP[w])
         assert (got[base:base + ke] == expect).all()
         assert (got[:base] == 0).all() and (got[base + ke:] == 0).all()
+
+
+@pytest.mark.parametrize("w", [1, 2, 4, 8])
+def test_pattern_compact(O, w):
+    rng = np.random.default_rng(w)
+    cases = [(0xB5000000, 8), (0xDEADBEEF, 32), (0x80000000, 1), (0x6C000000, 7), (0xFFFFFFFF, 32), (0x00000001, 32),
+             (0x12345678, 27), (0xF0000000, 5), (0x0, 8)]
+    for pat, L in cases:
+        for n in (0, 1, 7, 33, 1000, 65537, 1 << 20):
+            print(f"pattern w={w} pattern={pat:#x} L={L} n={n}", flush=True)
+            data = O.fill_values(n, NP[w], seed=n + L)
+            mask, _ = O.mask_pattern((n + 7) // 8, pat & (0xFFFFFFFF << (32 - L)) & 0xFFFFFFFF, L)
+            expect, ke = O.compact(data, mask, n)
+            assert csx.pattern_count(n, pat, L) == ke
+            canary = np.frombuffer(np.full((n + 8) * w, 0x3C, np.uint8).tobytes(), dtype=NP[w]).copy()
+            d_out = dev(canary)
+            d_cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+            _, k = csx.pattern_compact(dev(data) if n else torch.zeros(1, dtype=dev(canary).dtype, device="cuda"), pat, L, n=n, out=d_out,
+                                       count=d_cnt)
+            got = host(d_out, NP[w])
+            assert k == ke and int(d_cnt.item()) == ke
+            assert (got[:k] == expect).all() and (got[k:] == canary[k:]).all()
+    # the reference driver's own pattern (src/main.cu:54-64) at p = 0.5, L = 8
+    pat = O.pattern_from_selectivity(8, 0.5)
+    n = 300000
+    data = O.fill_values(n, NP[w], seed=5)
+    mask, _ = O.mask_pattern((n + 7) // 8, pat, 8)
+    expect, ke = O.compact(data, mask, n)
+    out, k = csx.pattern_compact(dev(data), pat, 8)
+    assert k == ke and (host(out, NP[w])[:k] == expect).all()
