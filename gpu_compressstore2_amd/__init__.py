@@ -87,6 +87,8 @@ SIGNATURES = {
     "csx_scan_u32": (_int, [_vp, _vp, _u64, _vp, _vp, _sz, _vp]),
     "csx_writeout_ex": (_int, [_vp, _vp]),
     "csx_copy": (_int, [_vp, _vp, _u64, _vp]),
+    "csx_streams_workspace_size": (_int, [_u64, _u32, _u32, _u32, ctypes.POINTER(_sz)]),
+    "csx_compact_streams": (_int, [_vp, _vp, _vp, _u64, _u32, _u32, _int, _vp, _u32, _vp, ctypes.POINTER(_u64), _vp, _sz]),
     "csx_gen_mask_bernoulli": (_int, [_vp, _u64, ctypes.c_double, _u64, _int, _vp]),
     "csx_gen_mask_pattern": (_int, [_vp, _u64, _u32, _u32, _int, _vp]),
     "csx_zipf_table": (_int, [ctypes.c_double, _u32, _vp, _vp]),
@@ -296,6 +298,24 @@ def pattern_compact(inp, pattern, pattern_length, n=None, out=None, count=None, 
     k = _u64(0)
     _check(lib().csx_pattern_compact(_ptr(inp), _ptr(out), n, inp.element_size(), pattern, pattern_length, _ptr(count), ctypes.byref(k),
                                      _stream(stream)))
+    return out, k.value
+
+
+def compact_streams(inp, mask, streams, n=None, chunk_len=1024, bit_order=MSB_FIRST, out=None, count=None, workspace=None):
+    """Multi-stream split over torch streams; synchronises streams[0] and returns (out, k)."""
+    _require_device(inp, mask, out, count)
+    n = inp.numel() if n is None else n
+    w = inp.element_size()
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=inp.dtype, device=inp.device)
+    need = _sz(0)
+    _check(lib().csx_streams_workspace_size(n, w, chunk_len, len(streams), ctypes.byref(need)))
+    if workspace is None:
+        workspace = alloc_workspace(need.value, inp.device)
+    arr = (ctypes.c_void_p * len(streams))(*[s.cuda_stream for s in streams])
+    k = _u64(0)
+    _check(lib().csx_compact_streams(_ptr(inp), _ptr(mask), _ptr(out), n, w, chunk_len, bit_order, arr, len(streams), _ptr(count),
+                                     ctypes.byref(k), _ptr(workspace), workspace.numel()))
     return out, k.value
 
 

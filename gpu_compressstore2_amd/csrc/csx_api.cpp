@@ -536,6 +536,134 @@ csx_status csx_pattern_compact(const void* d_in, void* d_out, uint64_t n, uint32
     return ok();
 }
 
+// ---------------------------------------------------------------- multi-stream split
+
+// Slices of max(tile, chunk) elements; per slice: its own scan header and
+// status words; shared: u64 base[S], u64 total[S], u32 counts and u64
+// offsets over all chunks (each slice uses its own range).
+struct StreamsPlan {
+    uint32_t S;
+    uint64_t unit, units, per_slice;
+    size_t base_off, total_off, counts_off, offsets_off, scan_off, scan_stride, total;
+    uint64_t begin(uint32_t i, uint64_t n) const
+    {
+        const uint64_t b = (uint64_t)i * per_slice * unit;
+        return b < n ? b : n;
+    }
+    uint64_t end(uint32_t i, uint64_t n) const { return i + 1 == S ? n : begin(i + 1, n); }
+};
+
+static StreamsPlan plan_streams(uint64_t n, uint32_t w, uint32_t chunk_len, uint32_t S)
+{
+    StreamsPlan p;
+    p.S = S;
+    p.unit = tile_elems(w) > chunk_len ? tile_elems(w) : chunk_len;
+    p.units = (n + p.unit - 1) / p.unit;
+    p.per_slice = p.units / S;
+    const uint64_t chunks = chunk_count(n, chunk_len);
+    // largest slice: the last one (it takes the remainder)
+    const uint64_t max_slice_chunks = (n - (uint64_t)(S - 1) * p.per_slice * p.unit + chunk_len - 1) / chunk_len;
+    p.base_off = 0;
+    p.total_off = align_up(8ull * S);
+    p.counts_off = align_up(p.total_off + 8ull * S);
+    p.offsets_off = align_up(p.counts_off + 4 * chunks);
+    p.scan_off = align_up(p.offsets_off + 8 * chunks);
+    p.scan_stride = align_up(sizeof(csx::WsHeader) + 8 * csx::scan_tiles(max_slice_chunks + 1));
+    p.total = p.scan_off + (size_t)S * p.scan_stride;
+    return p;
+}
+
+csx_status csx_streams_workspace_size(uint64_t n, uint32_t elem_size, uint32_t chunk_len, uint32_t stream_count, size_t* bytes)
+{
+    csx_status s;
+    if ((s = check_width(elem_size)) || (s = check_chunk(chunk_len)) || (s = check_n(n))) return s;
+    if (stream_count < 1 || stream_count > 64) return fail(CSX_ERR_INVALID, "stream count %u must be in [1, 64]", stream_count);
+    if (!bytes) return fail(CSX_ERR_INVALID, "null size pointer");
+    *bytes = plan_streams(n, elem_size, chunk_len, stream_count).total;
+    return ok();
+}
+
+csx_status csx_compact_streams(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, uint32_t chunk_len,
+                               int bit_order, const hipStream_t* streams, uint32_t stream_count, uint64_t* d_count, uint64_t* h_count, void* d_ws,
+                               size_t ws_bytes)
+{
+    csx_status s;
+    if ((s = check_width(elem_size)) || (s = check_chunk(chunk_len)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
+    if (stream_count < 1 || stream_count > 64 || !streams) return fail(CSX_ERR_INVALID, "need 1 to 64 streams");
+    const StreamsPlan p = plan_streams(n, elem_size, chunk_len, stream_count);
+    if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
+    if (n > 0 && (!d_in || !d_mask || !d_out)) return fail(CSX_ERR_INVALID, "null input, mask or output pointer");
+    if (((uintptr_t)d_in % elem_size) || ((uintptr_t)d_out % elem_size))
+        return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
+    auto base = static_cast<uint8_t*>(d_ws);
+    auto bases = reinterpret_cast<uint64_t*>(base + p.base_off);
+    auto totals = reinterpret_cast<uint64_t*>(base + p.total_off);
+    auto counts = reinterpret_cast<uint32_t*>(base + p.counts_off);
+    auto offsets = reinterpret_cast<uint64_t*>(base + p.offsets_off);
+    const uint32_t log2c = log2u(chunk_len);
+    auto in = static_cast<const uint8_t*>(d_in);
+    auto out = static_cast<uint8_t*>(d_out);
+
+    std::vector<hipEvent_t> carried(stream_count, nullptr), done(stream_count, nullptr);
+    auto cleanup = [&] {
+        for (auto e : carried)
+            if (e) (void)hipEventDestroy(e);
+        for (auto e : done)
+            if (e) (void)hipEventDestroy(e);
+    };
+    for (uint32_t i = 0; i < stream_count; i++) {
+        if (hipEventCreateWithFlags(&carried[i], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&done[i], hipEventDisableTiming) != hipSuccess) {
+            cleanup();
+            return fail(CSX_ERR_HIP, "event creation failed");
+        }
+    }
+    hipError_t e = hipSuccess;
+    for (uint32_t i = 0; i < stream_count && e == hipSuccess; i++) {
+        hipStream_t st = streams[i];
+        const uint64_t e0 = p.begin(i, n), e1 = p.end(i, n), ni = e1 - e0;
+        const uint64_t c0 = e0 >> log2c, ci = chunk_count(ni, chunk_len);
+        auto hdr = reinterpret_cast<csx::WsHeader*>(base + p.scan_off + i * p.scan_stride);
+        auto status = reinterpret_cast<uint64_t*>(base + p.scan_off + i * p.scan_stride + sizeof(csx::WsHeader));
+        // phases 1-2 of the slice run concurrently with the other slices
+        if (ni > 0) {
+            e = csx::launch_popc(d_mask + e0 / 8, counts + c0, ni, log2c, bit_order, st, hdr, status, csx::scan_tiles(ci));
+            if (e == hipSuccess) e = csx::launch_scan(counts + c0, offsets + c0, ci, totals + i, hdr, status, st, true);
+        }
+        else {
+            e = csx::launch_store_u64(totals + i, 0, st);
+        }
+        // the carry waits for the previous slice's carry (which follows its scan)
+        if (e == hipSuccess && i > 0) e = hipStreamWaitEvent(st, carried[i - 1], 0);
+        if (e == hipSuccess) e = csx::launch_carry(bases + i, bases, totals, i, st);
+        if (e == hipSuccess) e = hipEventRecord(carried[i], st);
+        if (e == hipSuccess && ni > 0) {
+            csx::TileArgs a{};
+            a.in = in + e0 * elem_size;
+            a.mask = d_mask + e0 / 8;
+            a.out = out;
+            a.n = ni;
+            a.offsets = offsets + c0;
+            a.log2c = log2c;
+            a.out_base = bases + i;
+            const bool aligned = (((uintptr_t)a.in | (uintptr_t)a.mask) & 15) == 0;
+            e = timed(st, [&] { return dispatch_tile(elem_size, csx::kModeWriteout, true, bit_order, aligned, a, st); });
+        }
+        if (e == hipSuccess) e = hipEventRecord(done[i], st);
+    }
+    // join on streams[0], then the total
+    for (uint32_t i = 1; i < stream_count && e == hipSuccess; i++) e = hipStreamWaitEvent(streams[0], done[i], 0);
+    uint64_t* count_dst = d_count ? d_count : &reinterpret_cast<csx::WsHeader*>(base + p.scan_off)->count;
+    if (e == hipSuccess) e = csx::launch_carry(count_dst, bases, totals, stream_count, streams[0]);
+    cleanup();
+    if (e != hipSuccess) return hip_fail(e, "csx_compact_streams");
+    if (h_count) {
+        CSX_HIP(hipMemcpyAsync(h_count, count_dst, sizeof(uint64_t), hipMemcpyDeviceToHost, streams[0]), "count copy");
+        CSX_HIP(hipStreamSynchronize(streams[0]), "stream synchronize");
+    }
+    return ok();
+}
+
 // ---------------------------------------------------------------- generators
 
 static uint64_t bernoulli_threshold(double p) { return (uint64_t)(p * UINT32_MAX); }

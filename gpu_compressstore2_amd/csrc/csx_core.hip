@@ -127,3 +127,23 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
 }
 
 } // namespace csx
+
+namespace csx {
+
+// base[i] = base[i-1] + total[i-1] (base[0] = 0): the carry of the multi-stream
+// split, one thread, after the previous slice's scan and carry.  The
+// reference's carry (src/kernels/kernel_streaming_add.cuh:8-11) races with a
+// read-modify-write on the same word from another stream; here every word has
+// one writer and the ordering comes from stream events.
+static __global__ void k_carry(uint64_t* dst, const uint64_t* base, const uint64_t* total, uint32_t i)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = i == 0 ? 0ull : base[i - 1] + total[i - 1];
+}
+
+hipError_t launch_carry(uint64_t* dst, const uint64_t* base, const uint64_t* total, uint32_t i, hipStream_t s)
+{
+    k_carry<<<1, 64, 0, s>>>(dst, base, total, i);
+    return hipGetLastError();
+}
+
+} // namespace csx

@@ -23,6 +23,8 @@ hipError_t launch_tree_offsets(const uint32_t* tree, uint32_t* out, uint64_t m, 
 hipError_t launch_store_u64(uint64_t* dst, uint64_t v, hipStream_t s);
 // both pointers 16-byte aligned
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);
+// *dst = base[i-1] + total[i-1] (0 for i = 0): the multi-stream carry
+hipError_t launch_carry(uint64_t* dst, const uint64_t* base, const uint64_t* total, uint32_t i, hipStream_t s);
 // device generators (csx_gen.hip)
 hipError_t launch_gen_bernoulli(uint8_t* mask, uint64_t n, uint64_t thr, uint64_t seed, int order, hipStream_t s);
 hipError_t launch_gen_pattern(uint8_t* mask, uint64_t n, uint32_t pattern, uint32_t L, int order, hipStream_t s);

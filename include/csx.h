@@ -178,6 +178,19 @@ csx_status csx_compact_w4(const uint32_t* d_in, const uint8_t* d_mask, uint32_t*
 csx_status csx_compact_w8(const uint64_t* d_in, const uint8_t* d_mask, uint64_t* d_out, uint64_t n, uint32_t chunk_len, int bit_order,
                           int pipeline, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream);
 
+/* Multi-stream split: the column is cut into stream_count slices of whole
+ * max(tile, chunk_len) units; slice i runs popcount, scan and writeout on
+ * streams[i], concurrently with the others, and its output offset is the
+ * carry base[i] = base[i-1] + total[i-1] computed on the device after an
+ * event from slice i-1 (one writer per word, ordered by events).  streams[0]
+ * waits for every slice at the end: synchronise on it.  The result equals
+ * csx_compact's.  Replaces bench3_3pass_streaming + its carry kernel
+ * (src/benchmarks.cuh:195-293, src/kernels/kernel_streaming_add.cuh:8-16). */
+csx_status csx_streams_workspace_size(uint64_t n, uint32_t elem_size, uint32_t chunk_len, uint32_t stream_count, size_t* bytes);
+csx_status csx_compact_streams(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, uint32_t chunk_len,
+                               int bit_order, const hipStream_t* streams, uint32_t stream_count, uint64_t* d_count, uint64_t* h_count, void* d_ws,
+                               size_t ws_bytes);
+
 /* Non-zero if a single-pass kernel that used this workspace gave up waiting
  * (never expected; reported as CSX_ERR_DEVICE by synchronous calls).
  * Synchronises the stream. */

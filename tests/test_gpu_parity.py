@@ -351,3 +351,22 @@ def test_pattern_compact(O, w):
     expect, ke = O.compact(data, mask, n)
     out, k = csx.pattern_compact(dev(data), pat, 8)
     assert k == ke and (host(out, NP[w])[:k] == expect).all()
+
+
+@pytest.mark.parametrize("nstreams", [1, 2, 3, 8])
+def test_compact_streams(O, nstreams):
+    streams = [torch.cuda.Stream() for _ in range(nstreams)]
+    for w, n, chunk in ((4, (1 << 20) + 77, 1024), (1, 3 * 32768 + 5, 4096), (8, 100, 32), (2, 0, 256), (4, 5 * 8192, 65536)):
+        print(f"streams={nstreams} w={w} n={n} chunk={chunk}", flush=True)
+        data = O.fill_values(n, NP[w], seed=n + 3)
+        mask = O.mask_bernoulli(n, 0.37, seed=n + 4)
+        expect, ke = O.compact(data, mask, n)
+        d_in = dev(data) if n else torch.zeros(1, dtype=dev(np.zeros(1, NP[w])).dtype, device="cuda")
+        d_mask = dev(mask) if n else torch.zeros(1, dtype=torch.uint8, device="cuda")
+        canary = np.frombuffer(np.full((n + 8) * w, 0x77, np.uint8).tobytes(), dtype=NP[w]).copy()
+        d_out = dev(canary)
+        torch.cuda.synchronize()
+        _, k = csx.compact_streams(d_in, d_mask, streams, n=n, chunk_len=chunk, out=d_out)
+        torch.cuda.synchronize()
+        got = host(d_out, NP[w])
+        assert k == ke and (got[:k] == expect).all() and (got[k:] == canary[k:]).all()
