@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import gpu_compressstore2_amd as csx  # noqa: E402
+from gpu_compressstore2_amd import baseline  # noqa: E402
 
 METRIC = "effective GB/s (data+mask+out) & % HBM peak, 2^28 u32 at 1/50/99% sel"
 HBM_PEAK_GBS = 8000.0
@@ -149,7 +150,18 @@ def main():
             del expect
             if name == "three_phase" and p == 0.5:
                 main_result = (gbs, ms / K, kern_avg, step_bytes)
-        del masks
+        # vendor baseline on the same inputs: rocPRIM flagged select
+        temp = torch.empty(max(baseline.temp_size(w, n), 16), dtype=torch.uint8, device="cuda")
+        ms = time_steps(lambda i: baseline.rocprim_select(datas[i % ROTATE], masks[i % ROTATE], n=n, out=out, count=count, temp=temp), K, W)
+        gbs = K * step_bytes / (ms * 1e-3) / 1e9
+        sweep[f"rocprim_select@{p}"] = {"GB/s": round(gbs, 1), "pct_peak": round(100 * gbs / HBM_PEAK_GBS, 2), "ms_per_step": round(ms / K, 5)}
+        print(f"[bench] rocprim_select p={p}: {gbs:.1f} GB/s ({ms / K:.4f} ms/step)", file=sys.stderr, flush=True)
+        r = (K - 1) % ROTATE
+        torch.cuda.synchronize()
+        if int(count.item()) != ks[r]:
+            print(f"[bench] rocPRIM count mismatch at p={p}", file=sys.stderr)
+            sys.exit(1)
+        del masks, temp
 
     # plain device copy of the input size (csx_copy): what this HBM sustains
     copy_ms = time_steps(lambda i: csx.copy(out, datas[i % ROTATE]), K, W)
@@ -200,6 +212,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "vs_rocprim_select": round(gbs / sweep["rocprim_select@0.5"]["GB/s"], 3),
         "sweep": sweep,
     }
     print(json.dumps(line), flush=True)

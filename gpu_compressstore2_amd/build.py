@@ -72,13 +72,22 @@ def build(jobs=None):
     jobs = jobs or min(len(units), max(1, (os.cpu_count() or 4)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, units))
-    tmp = LIB + ".tmp"
+    _link(LIB, objs)
+    # the rocPRIM baseline: a separate library, not part of the engine
+    _link(BASELINE_LIB, [_compile(("baseline_rocprim.hip", "baseline_rocprim.o", DEVICE))])
+    return LIB
+
+
+BASELINE_LIB = os.path.join(PKG, "libcsx_baseline.so")
+
+
+def _link(target, objs):
+    tmp = target + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    shutil.move(tmp, LIB)
-    return LIB
+    shutil.move(tmp, target)
 
 
 if __name__ == "__main__":

@@ -370,3 +370,15 @@ def test_compact_streams(O, nstreams):
         torch.cuda.synchronize()
         got = host(d_out, NP[w])
         assert k == ke and (got[:k] == expect).all() and (got[k:] == canary[k:]).all()
+
+
+def test_rocprim_baseline_agrees(O):
+    from gpu_compressstore2_amd import baseline
+
+    for w, n in ((4, (1 << 20) + 3), (1, 77777), (8, 4096)):
+        data = O.fill_values(n, NP[w], seed=w)
+        mask = O.mask_bernoulli(n, 0.5, seed=w + 1)
+        expect, ke = O.compact(data, mask, n)
+        out, cnt = baseline.rocprim_select(dev(data), dev(mask), n=n)
+        assert int(cnt.item()) == ke
+        assert (host(out, NP[w])[:ke] == expect).all()
