@@ -161,7 +161,18 @@ def main():
         if int(count.item()) != ks[r]:
             print(f"[bench] rocPRIM count mismatch at p={p}", file=sys.stderr)
             sys.exit(1)
-        del masks, temp
+        # ... and with the flags pre-expanded to one byte per element (not timed)
+        flags = [unpack_msb_first(m, n).to(torch.uint8) for m in masks]
+        temp = torch.empty(max(baseline.temp_size_bytes(w, n), 16), dtype=torch.uint8, device="cuda")
+        ms = time_steps(lambda i: baseline.rocprim_select_bytes(datas[i % ROTATE], flags[i % ROTATE], n=n, out=out, count=count, temp=temp), K, W)
+        gbs = K * step_bytes / (ms * 1e-3) / 1e9
+        sweep[f"rocprim_select_byteflags@{p}"] = {"GB/s": round(gbs, 1), "pct_peak": round(100 * gbs / HBM_PEAK_GBS, 2), "ms_per_step": round(ms / K, 5)}
+        print(f"[bench] rocprim_select_byteflags p={p}: {gbs:.1f} GB/s ({ms / K:.4f} ms/step)", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        if int(count.item()) != ks[r]:
+            print(f"[bench] rocPRIM byte-flag count mismatch at p={p}", file=sys.stderr)
+            sys.exit(1)
+        del masks, temp, flags
 
     # plain device copy of the input size (csx_copy): what this HBM sustains
     copy_ms = time_steps(lambda i: csx.copy(out, datas[i % ROTATE]), K, W)
@@ -213,6 +224,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "vs_rocprim_select": round(gbs / sweep["rocprim_select@0.5"]["GB/s"], 3),
+        "vs_rocprim_select_byteflags": round(gbs / sweep["rocprim_select_byteflags@0.5"]["GB/s"], 3),
         "sweep": sweep,
     }
     print(json.dumps(line), flush=True)

@@ -24,6 +24,10 @@ def lib():
         L.csxb_rocprim_select.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                           ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
         L.csxb_rocprim_select.restype = ctypes.c_int
+        L.csxb_rocprim_select_bytes_temp_size.argtypes = L.csxb_rocprim_select_temp_size.argtypes
+        L.csxb_rocprim_select_bytes_temp_size.restype = ctypes.c_int
+        L.csxb_rocprim_select_bytes.argtypes = L.csxb_rocprim_select.argtypes
+        L.csxb_rocprim_select_bytes.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -48,6 +52,31 @@ def rocprim_select(inp, mask, n=None, out=None, count=None, temp=None, stream=No
         temp = torch.empty(max(temp_size(inp.element_size(), n), 16), dtype=torch.uint8, device=inp.device)
     e = lib().csxb_rocprim_select(inp.element_size(), _ptr(inp), _ptr(mask), _ptr(out), _ptr(count), n, _ptr(temp), temp.numel(),
                                   _stream(stream))
+    if e != 0:
+        raise CsxError(3, f"rocprim select failed ({e})")
+    return out, count
+
+
+def temp_size_bytes(elem_size, n):
+    b = ctypes.c_size_t(0)
+    e = lib().csxb_rocprim_select_bytes_temp_size(elem_size, n, ctypes.byref(b))
+    if e != 0:
+        raise CsxError(3, f"rocprim select size query failed ({e})")
+    return b.value
+
+
+def rocprim_select_bytes(inp, flags_u8, n=None, out=None, count=None, temp=None, stream=None):
+    """rocprim::select with one flag byte per element (flags expanded beforehand)."""
+    _require_device(inp, flags_u8, out, count)
+    n = inp.numel() if n is None else n
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=inp.dtype, device=inp.device)
+    if count is None:
+        count = torch.zeros(1, dtype=torch.int64, device=inp.device)
+    if temp is None:
+        temp = torch.empty(max(temp_size_bytes(inp.element_size(), n), 16), dtype=torch.uint8, device=inp.device)
+    e = lib().csxb_rocprim_select_bytes(inp.element_size(), _ptr(inp), _ptr(flags_u8), _ptr(out), _ptr(count), n, _ptr(temp), temp.numel(),
+                                        _stream(stream))
     if e != 0:
         raise CsxError(3, f"rocprim select failed ({e})")
     return out, count

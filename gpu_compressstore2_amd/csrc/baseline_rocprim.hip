@@ -36,9 +36,46 @@ hipError_t dispatch(uint32_t w, void* temp, size_t& temp_bytes, const void* in, 
     }
 }
 
+template <typename T>
+hipError_t run_bytes(void* temp, size_t& temp_bytes, const void* in, const uint8_t* flags, void* out, unsigned long long* count, uint64_t n,
+                     hipStream_t s)
+{
+    return rocprim::select(temp, temp_bytes, static_cast<const T*>(in), flags, static_cast<T*>(out), count, (size_t)n, s);
+}
+
+hipError_t dispatch_bytes(uint32_t w, void* temp, size_t& temp_bytes, const void* in, const uint8_t* flags, void* out, unsigned long long* count,
+                          uint64_t n, hipStream_t s)
+{
+    switch (w) {
+        case 1: return run_bytes<uint8_t>(temp, temp_bytes, in, flags, out, count, n, s);
+        case 2: return run_bytes<uint16_t>(temp, temp_bytes, in, flags, out, count, n, s);
+        case 4: return run_bytes<uint32_t>(temp, temp_bytes, in, flags, out, count, n, s);
+        case 8: return run_bytes<uint64_t>(temp, temp_bytes, in, flags, out, count, n, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 } // namespace
 
 extern "C" {
+
+// the same select with one flag byte per element (the reference's unused
+// launch_cub_flagged_bytemask, src/cub_wraps.cuh:101-136); the byte flags are
+// expanded by the caller, outside any timing
+int csxb_rocprim_select_bytes_temp_size(uint32_t w, uint64_t n, size_t* bytes)
+{
+    size_t b = 0;
+    hipError_t e = dispatch_bytes(w, nullptr, b, nullptr, nullptr, nullptr, nullptr, n, 0);
+    *bytes = b;
+    return (int)e;
+}
+
+int csxb_rocprim_select_bytes(uint32_t w, const void* d_in, const uint8_t* d_flags, void* d_out, uint64_t* d_count, uint64_t n, void* d_temp,
+                              size_t temp_bytes, hipStream_t stream)
+{
+    size_t b = temp_bytes;
+    return (int)dispatch_bytes(w, d_temp, b, d_in, d_flags, d_out, reinterpret_cast<unsigned long long*>(d_count), n, stream);
+}
 
 // bytes of temporary storage rocprim::select needs for n elements of width w
 int csxb_rocprim_select_temp_size(uint32_t w, uint64_t n, size_t* bytes)
