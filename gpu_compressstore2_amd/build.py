@@ -75,10 +75,23 @@ def build(jobs=None):
     _link(LIB, objs)
     # the rocPRIM baseline: a separate library, not part of the engine
     _link(BASELINE_LIB, [_compile(("baseline_rocprim.hip", "baseline_rocprim.o", DEVICE))])
+    _build_driver()
     return LIB
 
 
 BASELINE_LIB = os.path.join(PKG, "libcsx_baseline.so")
+DRIVER = os.path.join(PKG, "csx_bench")
+
+
+def _build_driver():
+    """The main.cu-compatible benchmark driver, linked against both libraries."""
+    src = os.path.join(PKG, "driver", "csx_bench.cpp")
+    tmp = DRIVER + ".tmp"
+    cmd = [HIPCC, *COMMON, "-o", tmp, src, f"-L{PKG}", "-lcsx", "-lcsx_baseline", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    shutil.move(tmp, DRIVER)
 
 
 def _link(target, objs):
