@@ -87,6 +87,8 @@ SIGNATURES = {
     "csx_scan_u32": (_int, [_vp, _vp, _u64, _vp, _vp, _sz, _vp]),
     "csx_writeout_ex": (_int, [_vp, _vp]),
     "csx_copy": (_int, [_vp, _vp, _u64, _vp]),
+    "csx_batch_workspace_size": (_int, [_u32, _u64, _u32, ctypes.POINTER(_sz)]),
+    "csx_compact_batch": (_int, [_vp, _u32, _u64, _u32, _int, _vp, _vp, _sz, _vp]),
     "csx_streams_workspace_size": (_int, [_u64, _u32, _u32, _u32, ctypes.POINTER(_sz)]),
     "csx_compact_streams": (_int, [_vp, _vp, _vp, _u64, _u32, _u32, _int, _vp, _u32, _vp, ctypes.POINTER(_u64), _vp, _sz]),
     "csx_gen_mask_bernoulli": (_int, [_vp, _u64, ctypes.c_double, _u64, _int, _vp]),
@@ -317,6 +319,31 @@ def compact_streams(inp, mask, streams, n=None, chunk_len=1024, bit_order=MSB_FI
     _check(lib().csx_compact_streams(_ptr(inp), _ptr(mask), _ptr(out), n, w, chunk_len, bit_order, arr, len(streams), _ptr(count),
                                      ctypes.byref(k), _ptr(workspace), workspace.numel()))
     return out, k.value
+
+
+def segment_table(inputs, masks, outputs, lengths, device=None):
+    """Device table of csx_segment {in, mask, out, n} rows (int64 [nseg, 4])."""
+    rows = [[t_in.data_ptr(), t_m.data_ptr(), t_o.data_ptr(), int(n)] for t_in, t_m, t_o, n in zip(inputs, masks, outputs, lengths)]
+    return torch.tensor(rows, dtype=torch.int64, device=device or "cuda").reshape(-1, 4)
+
+
+def batch_workspace_size(nseg, max_total, elem_size):
+    out = _sz(0)
+    _check(lib().csx_batch_workspace_size(nseg, max_total, elem_size, ctypes.byref(out)))
+    return out.value
+
+
+def compact_batch(table, elem_size, max_total, counts=None, workspace=None, bit_order=MSB_FIRST, stream=None):
+    """Compact every column of a device segment table; returns the device counts."""
+    _require_device(table, counts)
+    nseg = table.shape[0]
+    if counts is None:
+        counts = torch.zeros(max(nseg, 1), dtype=torch.int64, device=table.device)
+    if workspace is None:
+        workspace = alloc_workspace(batch_workspace_size(nseg, max_total, elem_size), table.device)
+    _check(lib().csx_compact_batch(_ptr(table), nseg, max_total, elem_size, bit_order, _ptr(counts), _ptr(workspace), workspace.numel(),
+                                   _stream(stream)))
+    return counts
 
 
 VALUES_INTEGER = 0

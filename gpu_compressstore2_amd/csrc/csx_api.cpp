@@ -664,6 +664,58 @@ csx_status csx_compact_streams(const void* d_in, const uint8_t* d_mask, void* d_
     return ok();
 }
 
+// ---------------------------------------------------------------- batched columns
+
+static_assert(sizeof(csx_segment) == sizeof(csx::Segment), "segment layouts agree");
+
+struct BatchPlan {
+    uint64_t max_tiles, status_words;
+    size_t prefix_off, tcounts_off, goff_off, status_off, total;
+};
+
+static BatchPlan plan_batch(uint32_t nseg, uint64_t max_total, uint32_t w)
+{
+    BatchPlan p;
+    p.max_tiles = csx::batch_max_tiles(w, nseg, max_total);
+    p.status_words = csx::scan_tiles(p.max_tiles);
+    p.prefix_off = sizeof(csx::WsHeader);
+    p.tcounts_off = align_up(p.prefix_off + 8ull * (nseg + 1));
+    p.goff_off = align_up(p.tcounts_off + 4 * p.max_tiles);
+    p.status_off = align_up(p.goff_off + 8 * p.max_tiles);
+    p.total = align_up(p.status_off + 8 * (p.status_words + 1));
+    return p;
+}
+
+csx_status csx_batch_workspace_size(uint32_t nseg, uint64_t max_total_elements, uint32_t elem_size, size_t* bytes)
+{
+    csx_status s;
+    if ((s = check_width(elem_size)) || (s = check_n(max_total_elements))) return s;
+    if (!bytes) return fail(CSX_ERR_INVALID, "null size pointer");
+    *bytes = plan_batch(nseg, max_total_elements, elem_size).total;
+    return ok();
+}
+
+csx_status csx_compact_batch(const csx_segment* d_segments, uint32_t nseg, uint64_t max_total_elements, uint32_t elem_size, int bit_order,
+                             uint64_t* d_counts, void* d_ws, size_t ws_bytes, hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_width(elem_size)) || (s = check_order(bit_order)) || (s = check_n(max_total_elements))) return s;
+    if (nseg == 0) return ok();
+    if (!d_segments || !d_counts) return fail(CSX_ERR_INVALID, "null segment table or counts pointer");
+    const BatchPlan p = plan_batch(nseg, max_total_elements, elem_size);
+    if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
+    if (!launch_fits(p.max_tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)p.max_tiles);
+    auto base = static_cast<uint8_t*>(d_ws);
+    CSX_HIP(timed(stream, [&] {
+                return csx::launch_batch(elem_size, bit_order, reinterpret_cast<const csx::Segment*>(d_segments), nseg, p.max_tiles, d_counts,
+                                         reinterpret_cast<uint64_t*>(base + p.prefix_off), reinterpret_cast<uint32_t*>(base + p.tcounts_off),
+                                         reinterpret_cast<uint64_t*>(base + p.goff_off), reinterpret_cast<csx::WsHeader*>(base),
+                                         reinterpret_cast<uint64_t*>(base + p.status_off), stream);
+            }),
+            "csx_compact_batch");
+    return ok();
+}
+
 // ---------------------------------------------------------------- generators
 
 static uint64_t bernoulli_threshold(double p) { return (uint64_t)(p * UINT32_MAX); }

@@ -1,0 +1,182 @@
+// csx_batch.hip -- batched compaction of many independent columns.
+//
+// The columns are a device-resident table of segments {in, mask, out, n}.
+// Every launch reads its sizes from device memory, so the sequence needs no
+// host round trip and can be captured into a graph once and replayed with a
+// new table (the host only bounds the total for grid and workspace sizes):
+//   K0 plan    one workgroup: tile prefix over the segments, the total tile
+//              count, zeroed per-segment counts and look-back state
+//   K1 counts  persistent grid walking the (segment, tile) queue, one wave
+//              per tile: selected count of every tile
+//   K2 scan    the look-back scan over all tile counts (length from device)
+//   K3 write   one workgroup per (segment, tile): the tile engine with the
+//              tile's offset made local to its segment; the last tile of a
+//              segment writes the segment's selected count
+// The reference's closest pattern is its two-stream split with a carry
+// (src/benchmarks.cuh:195-293); here the split is by column, needing no carry.
+#include "csx_internal.hpp"
+#include "csx_kernels.hpp"
+
+namespace csx {
+
+constexpr uint32_t kPlanItems = 16;
+
+// tile_prefix[s] = tiles of segments [0, s); tile_prefix[nseg] = total
+__global__ __launch_bounds__(256) void k_batch_plan(const Segment* segs, uint32_t nseg, uint64_t te, uint64_t max_tiles, uint64_t* tile_prefix,
+                                                    uint64_t* counts, WsHeader* hdr, uint64_t* status, uint64_t status_words)
+{
+    __shared__ uint64_t scr[4];
+    const uint32_t t = threadIdx.x;
+    if (t == 0) {
+        hdr->ticket = 0;
+        hdr->error = 0;
+    }
+    for (uint64_t i = t; i < status_words; i += 256) status[i] = 0;
+    uint64_t run = 0;
+    for (uint64_t b = 0; b < nseg; b += 256 * kPlanItems) {
+        uint64_t v[kPlanItems];
+        uint64_t sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kPlanItems; k++) {
+            const uint64_t s = b + (uint64_t)t * kPlanItems + k;
+            v[k] = s < nseg ? (segs[s].n + te - 1) / te : 0;
+            if (s < nseg) counts[s] = 0;
+            sum += v[k];
+        }
+        uint64_t total;
+        const uint64_t ex = block_exclusive_sum<256>(sum, scr, &total);
+        uint64_t p = run + ex;
+#pragma unroll
+        for (uint32_t k = 0; k < kPlanItems; k++) {
+            const uint64_t s = b + (uint64_t)t * kPlanItems + k;
+            if (s < nseg) tile_prefix[s] = p;
+            p += v[k];
+        }
+        run += total;
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (run > max_tiles) {
+            hdr->error = 2;  // more tiles than the host bound: the launches below cover max_tiles only
+            run = max_tiles;
+        }
+        tile_prefix[nseg] = run;
+    }
+}
+
+// segment s with tile_prefix[s] <= g < tile_prefix[s + 1]
+__device__ __forceinline__ uint32_t find_segment(const uint64_t* tile_prefix, uint32_t nseg, uint64_t g)
+{
+    uint32_t lo = 0, hi = nseg;  // answer in [lo, hi)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (tile_prefix[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <int W, int ORDER>
+__global__ __launch_bounds__(256) void k_batch_counts(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, uint32_t* tcounts)
+{
+    constexpr uint64_t TE = kTileElems<W>;
+    constexpr uint32_t VEC_PER_TILE = (uint32_t)(TE / 128);
+    const uint32_t lane = lane_id();
+    const uint64_t total = tile_prefix[nseg];
+    const uint64_t wave = (blockIdx.x * 256ull + threadIdx.x) / kWave;
+    const uint64_t waves = gridDim.x * (256ull / kWave);
+    for (uint64_t g = wave; g < total; g += waves) {
+        const uint32_t s = find_segment(tile_prefix, nseg, g);
+        const Segment seg = segs[s];
+        const uint64_t lt = g - tile_prefix[s];
+        const uint64_t nbytes = (seg.n + 7) / 8;
+        const bool aligned = ((uintptr_t)seg.mask & 15) == 0;
+        uint32_t c = 0;
+        for (uint32_t v = lane; v < VEC_PER_TILE; v += kWave) {
+            const uint64_t vec = lt * VEC_PER_TILE + v;
+            const u32x4 x = aligned ? load_mask16<true>(seg.mask, vec * 16, nbytes) : load_mask16<false>(seg.mask, vec * 16, nbytes);
+            uint32_t w[4] = {to_lsb_word<ORDER>(x.x), to_lsb_word<ORDER>(x.y), to_lsb_word<ORDER>(x.z), to_lsb_word<ORDER>(x.w)};
+            const uint64_t e0 = vec * 128;
+            if (e0 + 128 > seg.n) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint64_t b = e0 + 32 * k;
+                    w[k] = b >= seg.n ? 0u : low_bits(w[k], (uint32_t)(seg.n - b));
+                }
+            }
+            c += popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
+        }
+        c = wave_sum(c);
+        if (lane == 0) tcounts[g] = c;
+    }
+}
+
+template <int W, int ORDER>
+__global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, const uint64_t* goff,
+                                                     uint64_t* counts)
+{
+    using C = TileCfg<W>;
+    __shared__ TileShared<C> sh;
+    __shared__ uint32_t s_seg;
+    const uint64_t g = blockIdx.x;
+    if (g >= tile_prefix[nseg]) return;  // surplus of the host's bound: uniform exit
+    if (threadIdx.x == 0) s_seg = find_segment(tile_prefix, nseg, g);
+    __syncthreads();
+    const uint32_t s = s_seg;
+    const Segment seg = segs[s];
+    const uint64_t first = tile_prefix[s];
+    TileArgs a{};
+    a.in = static_cast<const uint8_t*>(seg.in);
+    a.mask = seg.mask;
+    a.out = static_cast<uint8_t*>(seg.out);
+    a.n = seg.n;
+    a.given_off = goff[g] - goff[first];
+    a.count_out = counts + s;
+    if ((((uintptr_t)a.in) | ((uintptr_t)a.mask)) & 15)
+        tile_body<W, ORDER, false, kModeGiven, true, kTileVpt<W>>(a, sh, g - first);
+    else
+        tile_body<W, ORDER, true, kModeGiven, true, kTileVpt<W>>(a, sh, g - first);
+}
+
+uint64_t batch_max_tiles(uint32_t w, uint32_t nseg, uint64_t max_total)
+{
+    const uint64_t te = w == 1 ? kTileElems<1> : w == 2 ? kTileElems<2> : w == 4 ? kTileElems<4> : kTileElems<8>;
+    return max_total / te + nseg;
+}
+
+template <int W, int ORDER>
+static hipError_t batch_impl(const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix, uint32_t* tcounts,
+                             uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s)
+{
+    const uint64_t status_words = (max_tiles + kScanTile - 1) / kScanTile;
+    k_batch_plan<<<1, 256, 0, s>>>(segs, nseg, kTileElems<W>, max_tiles, tile_prefix, counts, hdr, status, status_words);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || max_tiles == 0) return e;
+    uint64_t g1 = (max_tiles + 3) / 4;
+    if (g1 > 4096) g1 = 4096;
+    k_batch_counts<W, ORDER><<<(unsigned)g1, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_scan<uint64_t><<<(unsigned)status_words, kScanBlock, 0, s>>>(tcounts, goff, max_tiles, nullptr, hdr, status, tile_prefix + nseg);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_batch_write<W, ORDER><<<(unsigned)max_tiles, 256, 0, s>>>(segs, nseg, tile_prefix, goff, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(uint32_t w, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
+                        uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s)
+{
+#define CSX_BATCH(W_)                                                                                   \
+    return order == kBitMsb ? batch_impl<W_, kBitMsb>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, s) \
+                            : batch_impl<W_, kBitLsb>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, s)
+    switch (w) {
+        case 1: CSX_BATCH(1);
+        case 2: CSX_BATCH(2);
+        case 4: CSX_BATCH(4);
+        default: CSX_BATCH(8);
+    }
+#undef CSX_BATCH
+}
+
+} // namespace csx

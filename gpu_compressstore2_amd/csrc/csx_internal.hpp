@@ -25,6 +25,10 @@ hipError_t launch_store_u64(uint64_t* dst, uint64_t v, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);
 // *dst = base[i-1] + total[i-1] (0 for i = 0): the multi-stream carry
 hipError_t launch_carry(uint64_t* dst, const uint64_t* base, const uint64_t* total, uint32_t i, hipStream_t s);
+// batched compaction (csx_batch.hip)
+uint64_t batch_max_tiles(uint32_t w, uint32_t nseg, uint64_t max_total);
+hipError_t launch_batch(uint32_t w, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
+                        uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s);
 // device generators (csx_gen.hip)
 hipError_t launch_gen_bernoulli(uint8_t* mask, uint64_t n, uint64_t thr, uint64_t seed, int order, hipStream_t s);
 hipError_t launch_gen_pattern(uint8_t* mask, uint64_t n, uint32_t pattern, uint32_t L, int order, hipStream_t s);

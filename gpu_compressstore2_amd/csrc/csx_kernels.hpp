@@ -144,7 +144,7 @@ constexpr int kScanTile = kScanBlock * kScanItems;
 // thread that later writes it, before the look-back.
 template <typename OutT>
 __global__ __launch_bounds__(kScanBlock) void k_scan(const uint32_t* counts, OutT* offsets, uint64_t m, uint64_t* __restrict__ total, WsHeader* hdr,
-                                                     uint64_t* status)
+                                                     uint64_t* status, const uint64_t* m_dev = nullptr)
 {
     __shared__ uint64_t s_scr[kScanBlock / kWave];
     __shared__ uint64_t s_tile;
@@ -152,7 +152,11 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const uint32_t* counts, Out
     if (threadIdx.x == 0) s_tile = atomicAdd(&hdr->ticket, 1u);
     __syncthreads();
     const uint64_t tile = s_tile;
+    // m_dev: the length lives in device memory; the grid covers an upper bound
+    // and the surplus workgroups leave here (uniformly, before any barrier)
+    if (m_dev) m = *m_dev;
     const uint64_t ntiles = (m + kScanTile - 1) / kScanTile;
+    if (tile >= ntiles) return;
     const uint64_t base = tile * kScanTile + threadIdx.x * (uint64_t)kScanItems;
 
     uint32_t v[kScanItems];
@@ -355,6 +359,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         // still publish its look-back status)
         if (tile_count == 0) {
             if (MODE == kModeTwoPass && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = prefix_total;
+            if (MODE == kModeGiven && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = a.given_off;
             return;
         }
     }
@@ -402,6 +407,13 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         if (t == 0) {
             sh.off = prefix_total;
             if (tile == ntiles - 1 && a.count_out) *a.count_out = prefix_total + tile_count;
+        }
+        __syncthreads();
+    }
+    else if constexpr (MODE == kModeGiven) {
+        if (t == 0) {
+            sh.off = a.given_off;
+            if (tile == ntiles - 1 && a.count_out) *a.count_out = a.given_off + tile_count;
         }
         __syncthreads();
     }

@@ -22,7 +22,8 @@ template <int W> constexpr uint64_t kTileElems = 256ull * kTileVpt<W> * (16ull /
 // Elements per tile of the chunk-count scan.
 constexpr uint64_t kScanTileElems = 256ull * 16ull;
 
-enum TileMode { kModeWriteout = 0, kModeFused = 1, kModeTwoPass = 2 };
+// kModeGiven: the tile's output offset is supplied in TileArgs::given_off
+enum TileMode { kModeWriteout = 0, kModeFused = 1, kModeTwoPass = 2, kModeGiven = 3 };
 
 // Arguments of one tile-engine launch.
 struct TileArgs {
@@ -44,6 +45,16 @@ struct TileArgs {
     const uint32_t* tcounts;
     const uint64_t* gsums;
     uint32_t log2g;
+    // given offset (kModeGiven)
+    uint64_t given_off;
+};
+
+// One column of a batched compaction (device-resident table).
+struct Segment {
+    const void* in;
+    const uint8_t* mask;
+    void* out;
+    uint64_t n;
 };
 
 // Tiles per group of the two-pass pipeline: a power of two >= 16 keeping

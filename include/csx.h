@@ -191,6 +191,28 @@ csx_status csx_compact_streams(const void* d_in, const uint8_t* d_mask, void* d_
                                int bit_order, const hipStream_t* streams, uint32_t stream_count, uint64_t* d_count, uint64_t* h_count, void* d_ws,
                                size_t ws_bytes);
 
+/* ---------------------------------------------------------------- batched columns
+ * Many independent columns in one call: d_segments is a DEVICE array of
+ * nseg {in, mask, out, n} descriptors; column s is compacted into its own
+ * out with d_counts[s] selected elements (device u64 array of nseg).
+ * Sizes are read on the device, so the call launches four kernels with no
+ * host round trip and can be captured into a graph once and replayed after
+ * rewriting the table.  max_total_elements bounds the sum of n (it sizes the
+ * grids and the workspace); a table exceeding it sets the workspace error
+ * word (csx_workspace_error) and leaves the excess uncompacted.  No
+ * counterpart in the reference beyond its two-stream split
+ * (src/benchmarks.cuh:195-293); MSB- or LSB-first masks, 1/2/4/8-byte
+ * elements, all columns of one width. */
+typedef struct {
+    const void* in;
+    const uint8_t* mask;
+    void* out;
+    uint64_t n;
+} csx_segment;
+csx_status csx_batch_workspace_size(uint32_t nseg, uint64_t max_total_elements, uint32_t elem_size, size_t* bytes);
+csx_status csx_compact_batch(const csx_segment* d_segments, uint32_t nseg, uint64_t max_total_elements, uint32_t elem_size, int bit_order,
+                             uint64_t* d_counts, void* d_ws, size_t ws_bytes, hipStream_t stream);
+
 /* Non-zero if a single-pass kernel that used this workspace gave up waiting
  * (never expected; reported as CSX_ERR_DEVICE by synchronous calls).
  * Synchronises the stream. */

@@ -1,0 +1,109 @@
+"""Batched compaction of many columns (csx_compact_batch): every column
+against the oracle, and graph capture with replays on new inputs."""
+import numpy as np
+import pytest
+import torch
+
+import gpu_compressstore2_amd as csx
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+NP = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
+VIEW = {1: np.uint8, 2: np.int16, 4: np.int32, 8: np.int64}
+TILE = {1: 32768, 2: 16384, 4: 8192, 8: 4096}
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(VIEW[a.dtype.itemsize])).cuda()
+
+
+def build(O, w, lengths, seed, order=oracle.MSB_FIRST):
+    rng = np.random.default_rng(seed)
+    cols = []
+    for i, n in enumerate(lengths):
+        data = O.fill_values(n, NP[w], seed=seed * 1000 + i)
+        p = float(rng.uniform(0.01, 0.99))
+        mask = O.mask_bernoulli(n, p, seed=seed * 1000 + i + 7, bit_order=order)
+        expect, k = O.compact(data, mask, n, order)
+        cols.append((data, mask, expect, k))
+    ins = [dev(c[0]) if c[0].size else torch.zeros(1, dtype=dev(np.zeros(1, NP[w])).dtype, device="cuda") for c in cols]
+    masks = [dev(c[1]) if c[1].size else torch.zeros(1, dtype=torch.uint8, device="cuda") for c in cols]
+    outs = [torch.full((max(n, 1) + 4,), -1, dtype=ins[0].dtype, device="cuda") for n in lengths]
+    return cols, ins, masks, outs
+
+
+def check(cols, outs, counts, w):
+    counts = counts.cpu().numpy()
+    for i, (data, mask, expect, k) in enumerate(cols):
+        assert counts[i] == k, (i, counts[i], k)
+        got = outs[i].cpu().numpy().view(NP[w])
+        assert (got[:k] == expect).all(), i
+        assert (got[k:].view(np.uint8) == 0xFF).all(), i  # nothing written past the count
+
+
+@pytest.mark.parametrize("order", [oracle.MSB_FIRST, oracle.LSB_FIRST])
+@pytest.mark.parametrize("w", [1, 2, 4, 8])
+def test_batch_parity(O, w, order):
+    T = TILE[w]
+    lengths = [0, 1, 7, 8, 100, T - 1, T, T + 1, 3 * T + 5, 1000, 0, 65536, 250003, 31, 2 * T]
+    print(f"batch w={w} order={order} nseg={len(lengths)}", flush=True)
+    cols, ins, masks, outs = build(O, w, lengths, seed=w + 10 * order, order=order)
+    table = csx.segment_table(ins, masks, outs, lengths)
+    counts = csx.compact_batch(table, w, sum(lengths), bit_order=order)
+    torch.cuda.synchronize()
+    check(cols, outs, counts, w)
+
+
+def test_batch_many_log_uniform_columns(O):
+    rng = np.random.default_rng(4)
+    lengths = [int(x) for x in np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 16), 300))]
+    cols, ins, masks, outs = build(O, 4, lengths, seed=3)
+    table = csx.segment_table(ins, masks, outs, lengths)
+    counts = csx.compact_batch(table, 4, sum(lengths))
+    torch.cuda.synchronize()
+    check(cols, outs, counts, 4)
+
+
+def test_batch_bound_too_small_sets_error(O):
+    lengths = [50000, 50000]
+    cols, ins, masks, outs = build(O, 4, lengths, seed=5)
+    table = csx.segment_table(ins, masks, outs, lengths)
+    ws = csx.alloc_workspace(csx.batch_workspace_size(2, 10, 4))
+    csx.compact_batch(table, 4, 10, workspace=ws)
+    assert csx.workspace_error(ws) == 2
+
+
+def test_batch_graph_capture_replays(O):
+    lengths = [5000, 70000, 123, 9000, 0, 40000]
+    cols, ins, masks, outs = build(O, 4, lengths, seed=8)
+    table = csx.segment_table(ins, masks, outs, lengths)
+    counts = torch.zeros(len(lengths), dtype=torch.int64, device="cuda")
+    ws = csx.alloc_workspace(csx.batch_workspace_size(len(lengths), sum(lengths), 4))
+    csx.compact_batch(table, 4, sum(lengths), counts=counts, workspace=ws)  # eager, warms up
+    torch.cuda.synchronize()
+    check(cols, outs, counts, 4)
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            csx.compact_batch(table, 4, sum(lengths), counts=counts, workspace=ws, stream=s)
+    torch.cuda.synchronize()
+    for rep in range(4):
+        # new inputs in the same buffers, outputs reset, then replay
+        new = []
+        for i, n in enumerate(lengths):
+            data = O.fill_values(n, np.uint32, seed=500 + 10 * rep + i)
+            mask = O.mask_bernoulli(n, 0.2 + 0.15 * rep, seed=600 + 10 * rep + i)
+            expect, k = O.compact(data, mask, n)
+            if n:
+                ins[i].copy_(dev(data))
+                masks[i].copy_(dev(mask))
+            outs[i].fill_(-1)
+            new.append((data, mask, expect, k))
+        counts.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        check(new, outs, counts, 4)
