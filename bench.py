@@ -77,6 +77,98 @@ def time_steps(fn, steps, warmup):
     return a.elapsed_time(b)
 
 
+def bench_batch(steps, warmup, seed=2024, nseg=4096):
+    """4096 u32 columns, lengths log-uniform in [2^10, 2^24], per-column
+    Bernoulli selectivity uniform in [0.01, 0.99]: one batched call against
+    one csx_compact call per column (both eager, and both graph-replayed)."""
+    import ctypes
+
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    lengths = np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 24), nseg)).astype(np.int64)
+    sels = rng.uniform(0.01, 0.99, nseg)
+    starts = np.zeros(nseg, np.int64)
+    mstarts = np.zeros(nseg, np.int64)
+    pos = mpos = 0
+    for i, n in enumerate(lengths):
+        starts[i], mstarts[i] = pos, mpos
+        pos += (int(n) + 63) // 64 * 64           # 256-byte aligned column starts
+        mpos += ((int(n) + 7) // 8 + 255) // 256 * 256
+    total = int(lengths.sum())
+    data = csx.gen_values(pos, 4, csx.VALUES_INTEGER, seed=seed)
+    out = torch.empty(pos, dtype=torch.int32, device="cuda")
+    masks = torch.empty(mpos, dtype=torch.uint8, device="cuda")
+    ins, ms, outs = [], [], []
+    for i in range(nseg):
+        n = int(lengths[i])
+        mv = masks[mstarts[i]: mstarts[i] + (n + 7) // 8]
+        csx.gen_mask_bernoulli(n, float(sels[i]), seed=seed + i, out=mv)
+        ins.append(data[starts[i]: starts[i] + n])
+        ms.append(mv)
+        outs.append(out[starts[i]: starts[i] + n])
+    table = csx.segment_table(ins, ms, outs, lengths)
+    counts = torch.zeros(nseg, dtype=torch.int64, device="cuda")
+    ws = csx.alloc_workspace(csx.batch_workspace_size(nseg, total, 4))
+    csx.compact_batch(table, 4, total, counts=counts, workspace=ws)
+    torch.cuda.synchronize()
+    ks = counts.cpu().numpy()
+    step_bytes = float(4 * total + total / 8 + 4 * ks.sum())
+    res = {"columns": nseg, "total_elements": total, "algorithmic_bytes": int(step_bytes)}
+
+    ms_b = time_steps(lambda i: csx.compact_batch(table, 4, total, counts=counts, workspace=ws), steps, warmup)
+    res["batched_GB/s"] = round(steps * step_bytes / (ms_b * 1e-3) / 1e9, 1)
+    res["batched_ms"] = round(ms_b / steps, 4)
+
+    # one csx_compact per column, through the C ABI with prebuilt arguments
+    L = csx.lib()
+    cws = csx.alloc_workspace(csx.workspace_size(int(lengths.max()), 4, 1024))
+    one = torch.empty(1, dtype=torch.int64, device="cuda")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    args = [(ctypes.c_void_p(ins[i].data_ptr()), ctypes.c_void_p(ms[i].data_ptr()), ctypes.c_void_p(outs[i].data_ptr()), int(lengths[i]))
+            for i in range(nseg)]
+    wsp, wsn, cnt = ctypes.c_void_p(cws.data_ptr()), cws.numel(), ctypes.c_void_p(one.data_ptr())
+
+    def per_column(_i, stream=stream):
+        for a_in, a_m, a_o, n in args:
+            L.csx_compact(a_in, a_m, a_o, n, 4, 1024, 0, csx.PIPELINE_THREE_PHASE, cnt, None, wsp, wsn, stream)
+
+    ms_c = time_steps(per_column, max(2, steps // 5), 1)
+    res["per_column_GB/s"] = round(max(2, steps // 5) * step_bytes / (ms_c * 1e-3) / 1e9, 1)
+    res["per_column_ms"] = round(ms_c / max(2, steps // 5), 4)
+
+    # both forms captured into graphs (no host launch cost at all)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    gb, gc = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(gb, stream=s):
+            csx.compact_batch(table, 4, total, counts=counts, workspace=ws, stream=s)
+        with torch.cuda.graph(gc, stream=s):
+            per_column(0, stream=ctypes.c_void_p(s.cuda_stream))
+    torch.cuda.synchronize()
+    ms_gb = time_steps(lambda i: gb.replay(), steps, warmup)
+    ms_gc = time_steps(lambda i: gc.replay(), steps, warmup)
+    res["batched_graph_GB/s"] = round(steps * step_bytes / (ms_gb * 1e-3) / 1e9, 1)
+    res["per_column_graph_GB/s"] = round(steps * step_bytes / (ms_gc * 1e-3) / 1e9, 1)
+    res["speedup_vs_per_column"] = round(ms_c / max(2, steps // 5) / (ms_b / steps), 2)
+    res["speedup_vs_per_column_graph"] = round(ms_gc / ms_gb, 2)
+    # correctness of the replayed batch: three columns against torch's masked_select
+    csx.compact_batch(table, 4, total, counts=counts, workspace=ws)
+    torch.cuda.synchronize()
+    for i in (0, nseg // 2, int(np.argmax(lengths))):
+        n = int(lengths[i])
+        expect = torch.masked_select(ins[i], unpack_msb_first(ms[i], n))
+        k = int(counts[i].item())
+        if k != expect.numel() or not torch.equal(outs[i][:k], expect):
+            print(f"[bench] batch MISMATCH in column {i}", file=sys.stderr)
+            sys.exit(1)
+    print(f"[bench] batch: {res}", file=sys.stderr, flush=True)
+    del data, out, masks, table, ws, cws, gb, gc
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +177,7 @@ def main():
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--chunk", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 4096-column batched measurement")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline")
     ap.add_argument("--sels", default="0.01,0.5,0.99", help="selectivities to sweep (0.5 is always included)")
     ap.add_argument("--pipelines", default="three_phase,two_pass,fused", help="pipelines to sweep (three_phase is always included)")
@@ -173,6 +266,12 @@ def main():
             print(f"[bench] rocPRIM byte-flag count mismatch at p={p}", file=sys.stderr)
             sys.exit(1)
         del masks, temp, flags
+
+    if not args.no_batch:
+        del datas
+        torch.cuda.empty_cache()
+        sweep["batch_4096_columns"] = bench_batch(args.steps, args.warmup)
+        datas = [make_data(n, 100 + r) for r in range(ROTATE)]
 
     # plain device copy of the input size (csx_copy): what this HBM sustains
     copy_ms = time_steps(lambda i: csx.copy(out, datas[i % ROTATE]), K, W)
