@@ -42,6 +42,7 @@ def _units():
     units = [
         ("csx_core.hip", "csx_core.o", DEVICE),
         ("csx_api.cpp", "csx_api.o", []),
+        ("csx_server.cpp", "csx_server.o", []),
     ]
     for w in (1, 2, 4, 8):
         units.append(("csx_tile.hip", f"csx_tile_w{w}.o", DEVICE + [f"-DCSX_TILE_W={w}"]))
@@ -83,15 +84,18 @@ BASELINE_LIB = os.path.join(PKG, "libcsx_baseline.so")
 DRIVER = os.path.join(PKG, "csx_bench")
 
 
+SERVE = os.path.join(PKG, "csx_serve")
+
+
 def _build_driver():
-    """The main.cu-compatible benchmark driver, linked against both libraries."""
-    src = os.path.join(PKG, "driver", "csx_bench.cpp")
-    tmp = DRIVER + ".tmp"
-    cmd = [HIPCC, *COMMON, "-o", tmp, src, f"-L{PKG}", "-lcsx", "-lcsx_baseline", "-Wl,-rpath,$ORIGIN"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    shutil.move(tmp, DRIVER)
+    """The main.cu-compatible benchmark driver and the serving-loop benchmark."""
+    for src, target, libs in (("csx_bench.cpp", DRIVER, ["-lcsx", "-lcsx_baseline"]), ("csx_serve.cpp", SERVE, ["-lcsx"])):
+        tmp = target + ".tmp"
+        cmd = [HIPCC, *COMMON, f"--offload-arch={ARCH}", "-o", tmp, os.path.join(PKG, "driver", src), f"-L{PKG}", *libs, "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        shutil.move(tmp, target)
 
 
 def _link(target, objs):

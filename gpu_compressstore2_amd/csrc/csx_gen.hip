@@ -164,27 +164,34 @@ __global__ __launch_bounds__(256) void k_gen_runs_write(uint8_t* mask, const uin
 }
 
 // ---------------------------------------------------------------- values
+// each thread produces a run of kValueRun consecutive elements after one jump
+constexpr uint64_t kValueRun = 16;
+
 template <int W>
 __global__ __launch_bounds__(256) void k_gen_values(uint8_t* out, uint64_t n, int kind, uint64_t st0)
 {
-    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
-        if constexpr (W == 8) {
-            uint64_t st = pcg_advance(st0, 2 * e);
-            const uint64_t hi = pcg_next(st), lo = pcg_next(st);
-            uint64_t v = (hi << 32) | lo;
-            if (kind == 1) {
-                const double d = (double)(v >> 11) * (1.0 / 9007199254740992.0);
-                v = __double_as_longlong(d);
+    const uint64_t runs = (n + kValueRun - 1) / kValueRun;
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < runs; r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t e0 = r * kValueRun;
+        const uint64_t e1 = e0 + kValueRun < n ? e0 + kValueRun : n;
+        uint64_t st = pcg_advance(st0, (W == 8 ? 2 : 1) * e0);
+        for (uint64_t e = e0; e < e1; e++) {
+            if constexpr (W == 8) {
+                const uint64_t hi = pcg_next(st), lo = pcg_next(st);
+                uint64_t v = (hi << 32) | lo;
+                if (kind == 1) {
+                    const double d = (double)(v >> 11) * (1.0 / 9007199254740992.0);
+                    v = __double_as_longlong(d);
+                }
+                reinterpret_cast<uint64_t*>(out)[e] = v;
             }
-            reinterpret_cast<uint64_t*>(out)[e] = v;
-        }
-        else {
-            uint64_t st = pcg_advance(st0, e);
-            uint32_t v = pcg_next(st);
-            if (W == 4 && kind == 1) v = __float_as_uint((float)(v >> 8) * (1.0f / 16777216.0f));
-            if constexpr (W == 4) reinterpret_cast<uint32_t*>(out)[e] = v;
-            if constexpr (W == 2) reinterpret_cast<uint16_t*>(out)[e] = (uint16_t)v;
-            if constexpr (W == 1) out[e] = (uint8_t)v;
+            else {
+                uint32_t v = pcg_next(st);
+                if (W == 4 && kind == 1) v = __float_as_uint((float)(v >> 8) * (1.0f / 16777216.0f));
+                if constexpr (W == 4) reinterpret_cast<uint32_t*>(out)[e] = v;
+                if constexpr (W == 2) reinterpret_cast<uint16_t*>(out)[e] = (uint16_t)v;
+                if constexpr (W == 1) out[e] = (uint8_t)v;
+            }
         }
     }
 }
@@ -258,7 +265,7 @@ hipError_t launch_gen_runs(uint8_t* mask, uint64_t n, uint32_t thr, uint64_t see
 hipError_t launch_gen_values(void* out, uint64_t n, uint32_t w, int kind, uint64_t seed, hipStream_t s)
 {
     const uint64_t st0 = splitmix_seed(seed);
-    const unsigned g = gen_grid(n);
+    const unsigned g = gen_grid((n + kValueRun - 1) / kValueRun);
     auto o = static_cast<uint8_t*>(out);
     switch (w) {
         case 1: k_gen_values<1><<<g, 256, 0, s>>>(o, n, kind, st0); break;

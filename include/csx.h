@@ -213,6 +213,56 @@ csx_status csx_batch_workspace_size(uint32_t nseg, uint64_t max_total_elements, 
 csx_status csx_compact_batch(const csx_segment* d_segments, uint32_t nseg, uint64_t max_total_elements, uint32_t elem_size, int bit_order,
                              uint64_t* d_counts, void* d_ws, size_t ws_bytes, hipStream_t stream);
 
+/* ---------------------------------------------------------------- serving
+ * Device memory arena: one allocation carved first-fit; the HBM budget of
+ * the serving loop.  csx_arena_alloc returns CSX_ERR_WORKSPACE when no free
+ * range fits (the caller keeps the request waiting).  Not thread-safe
+ * against csx_arena_destroy. */
+typedef struct csx_arena csx_arena;
+csx_status csx_arena_create(uint64_t bytes, csx_arena** arena);
+csx_status csx_arena_destroy(csx_arena* arena);
+csx_status csx_arena_alloc(csx_arena* arena, uint64_t bytes, void** d_ptr);
+csx_status csx_arena_free(csx_arena* arena, void* d_ptr);
+uint64_t csx_arena_used(const csx_arena* arena);
+
+/* Continuous-batching server.  csx_server_submit queues a request (device
+ * buffers, MSB-first mask) on the host; csx_server_step launches, while one
+ * of `streams` slots is free, one batched compaction (csx_compact_batch) of
+ * queued requests of one element width (at most max_batch requests and
+ * max_batch_elements elements), so new arrivals run while earlier batches are
+ * still on the device; csx_server_poll retires finished batches without
+ * blocking and reports, per request, the caller's id, the selected count and
+ * the latency from submission to that poll.  The results equal csx_compact's
+ * on each request alone.  One host thread drives a server.  No counterpart in
+ * the reference. */
+typedef struct csx_server csx_server;
+typedef struct {
+    uint32_t streams;            /* concurrent batches (1..16) */
+    uint32_t max_batch;          /* requests per batch */
+    uint64_t max_batch_elements; /* elements per batch (also the largest request) */
+} csx_server_config;
+typedef struct {
+    uint64_t id;
+    const void* in;
+    const uint8_t* mask;
+    void* out;
+    uint64_t n;
+    uint32_t elem_size;
+    hipEvent_t ready; /* nullable: the batch stream waits for it (inputs produced on another stream) */
+} csx_request;
+typedef struct {
+    uint64_t id;
+    uint64_t count;
+    double latency_ms;
+} csx_completion;
+csx_status csx_server_create(const csx_server_config* config, csx_server** server);
+csx_status csx_server_destroy(csx_server* server);
+csx_status csx_server_submit(csx_server* server, const csx_request* request);
+csx_status csx_server_step(csx_server* server, uint32_t* launched);
+csx_status csx_server_poll(csx_server* server, csx_completion* done, uint32_t capacity, uint32_t* count);
+uint32_t csx_server_queued(const csx_server* server);
+uint32_t csx_server_inflight(const csx_server* server);
+
 /* Non-zero if a single-pass kernel that used this workspace gave up waiting
  * (never expected; reported as CSX_ERR_DEVICE by synchronous calls).
  * Synchronises the stream. */
