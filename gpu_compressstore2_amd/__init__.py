@@ -102,6 +102,19 @@ SIGNATURES = {
     "csx_pattern_compact": (_int, [_vp, _vp, _u64, _u32, _u32, _u32, _vp, ctypes.POINTER(_u64), _vp]),
     "csx_profile_begin": (_int, []),
     "csx_profile_end": (_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
+    # serving: struct arguments travel as void* (structs in serving.py)
+    "csx_arena_create": (_int, [_u64, ctypes.POINTER(_vp)]),
+    "csx_arena_destroy": (_int, [_vp]),
+    "csx_arena_alloc": (_int, [_vp, _u64, ctypes.POINTER(_vp)]),
+    "csx_arena_free": (_int, [_vp, _vp]),
+    "csx_arena_used": (_u64, [_vp]),
+    "csx_server_create": (_int, [_vp, ctypes.POINTER(_vp)]),
+    "csx_server_destroy": (_int, [_vp]),
+    "csx_server_submit": (_int, [_vp, _vp]),
+    "csx_server_step": (_int, [_vp, ctypes.POINTER(_u32)]),
+    "csx_server_poll": (_int, [_vp, _vp, _u32, ctypes.POINTER(_u32)]),
+    "csx_server_queued": (_u32, [_vp]),
+    "csx_server_inflight": (_u32, [_vp]),
 }
 
 _LIB = None

@@ -2,8 +2,6 @@
 libcsx.so (csx_server_*, csx_arena_*).  The scheduling itself is C++."""
 import ctypes
 
-import torch
-
 from . import CsxError, _check, lib
 
 _vp = ctypes.c_void_p
@@ -24,26 +22,7 @@ class Completion(ctypes.Structure):
 
 
 def _bind():
-    L = lib()
-    sig = {
-        "csx_arena_create": (ctypes.c_int, [_u64, ctypes.POINTER(_vp)]),
-        "csx_arena_destroy": (ctypes.c_int, [_vp]),
-        "csx_arena_alloc": (ctypes.c_int, [_vp, _u64, ctypes.POINTER(_vp)]),
-        "csx_arena_free": (ctypes.c_int, [_vp, _vp]),
-        "csx_arena_used": (_u64, [_vp]),
-        "csx_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
-        "csx_server_destroy": (ctypes.c_int, [_vp]),
-        "csx_server_submit": (ctypes.c_int, [_vp, ctypes.POINTER(Request)]),
-        "csx_server_step": (ctypes.c_int, [_vp, ctypes.POINTER(_u32)]),
-        "csx_server_poll": (ctypes.c_int, [_vp, _vp, _u32, ctypes.POINTER(_u32)]),
-        "csx_server_queued": (_u32, [_vp]),
-        "csx_server_inflight": (_u32, [_vp]),
-    }
-    for name, (res, args) in sig.items():
-        f = getattr(L, name)
-        f.restype = res
-        f.argtypes = args
-    return L
+    return lib()
 
 
 class Arena:
@@ -82,13 +61,13 @@ class Server:
         self.L = _bind()
         self.h = _vp()
         cfg = ServerConfig(streams, max_batch, max_batch_elements)
-        _check(self.L.csx_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
+        _check(self.L.csx_server_create(_vp(ctypes.addressof(cfg)), ctypes.byref(self.h)))
         self._done = (Completion * 4096)()
 
     def submit(self, rid, inp, mask, out, n=None, ready=None):
         n = inp.numel() if n is None else n
         r = Request(rid, inp.data_ptr(), mask.data_ptr(), out.data_ptr(), n, inp.element_size(), ready)
-        _check(self.L.csx_server_submit(self.h, ctypes.byref(r)))
+        _check(self.L.csx_server_submit(self.h, _vp(ctypes.addressof(r))))
 
     def step(self):
         k = _u32(0)
@@ -119,4 +98,4 @@ class Server:
             self.h = None
 
 
-__all__ = ["Arena", "Server", "CsxError", "torch"]
+__all__ = ["Arena", "Server", "ServerConfig", "Request", "Completion"]
