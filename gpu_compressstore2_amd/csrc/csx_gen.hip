@@ -16,6 +16,7 @@
 //               (generate_mask_pattern, src/data_generator.cuh:113-128)
 #include "csx_internal.hpp"
 #include "csx_device.hpp"
+#include <csx.h>
 
 namespace csx {
 
@@ -38,8 +39,14 @@ __device__ __forceinline__ uint32_t pcg_next(uint64_t& st)
     return (xorshifted >> rot) | (xorshifted << ((-rot) & 31));
 }
 
-// state after `delta` steps from `st`
-__device__ inline uint64_t pcg_advance(uint64_t st, uint64_t delta)
+// The LCG step s -> M*s + 1 taken `delta` times is again affine,
+// s -> mult*s + plus; maps of one LCG commute, so a jump applies the maps of
+// 2^i steps for the set bits of delta in any order.
+struct Affine {
+    uint64_t mult, plus;
+};
+
+__host__ __device__ constexpr Affine lcg_steps(uint64_t delta)
 {
     uint64_t cur_mult = kPcgMult, cur_plus = 1ull, acc_mult = 1ull, acc_plus = 0ull;
     while (delta) {
@@ -51,7 +58,31 @@ __device__ inline uint64_t pcg_advance(uint64_t st, uint64_t delta)
         cur_mult *= cur_mult;
         delta >>= 1;
     }
-    return acc_mult * st + acc_plus;
+    return Affine{acc_mult, acc_plus};
+}
+
+// maps of 2^i steps, i < 64, fixed at compile time
+struct JumpTable {
+    uint64_t mult[64], plus[64];
+    constexpr JumpTable() : mult(), plus()
+    {
+        uint64_t m = kPcgMult, p = 1ull;
+        for (int i = 0; i < 64; i++) {
+            mult[i] = m;
+            plus[i] = p;
+            p = (m + 1ull) * p;
+            m = m * m;
+        }
+    }
+};
+__constant__ const JumpTable kJump{};
+
+// state after `delta` steps from `st`: one multiply-add per set bit
+__device__ inline uint64_t pcg_advance(uint64_t st, uint64_t delta)
+{
+    for (int i = 0; delta >> i; i++)
+        if ((delta >> i) & 1ull) st = kJump.mult[i] * st + kJump.plus[i];
+    return st;
 }
 
 // store 32 mask bits (LSB-first word of elements 32*wi ..) in the caller's order,
@@ -76,14 +107,21 @@ __device__ __forceinline__ uint32_t valid_bits(uint64_t wi, uint64_t n)
 }
 
 // ---------------------------------------------------------------- bernoulli
-template <int ORDER> __global__ __launch_bounds__(256) void k_gen_bernoulli(uint8_t* mask, uint64_t n, uint64_t thr, uint64_t st0)
+// Lane t of S lanes makes words t, t+S, t+2S, ...: one jump to draw 32t, 32
+// sequential steps per word, then `skip` (the map of 32*(S-1) steps) to the
+// next word of the lane.  Consecutive lanes store consecutive words.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_gen_bernoulli(uint8_t* mask, uint64_t n, uint64_t thr, uint64_t st0, Affine skip)
 {
     const uint64_t nwords = (n + 31) / 32, nbytes = (n + 7) / 8;
-    for (uint64_t wi = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; wi < nwords; wi += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t st = pcg_advance(st0, wi * 32);
+    const uint64_t wi0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (wi0 >= nwords) return;
+    uint64_t st = pcg_advance(st0, wi0 * 32);
+    for (uint64_t wi = wi0; wi < nwords; wi += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t word = 0;
         for (int k = 0; k < 32; k++) word |= (uint32_t)((uint64_t)pcg_next(st) < thr) << k;
         store_mask_word<ORDER>(mask, wi, word & valid_bits(wi, n), nbytes);
+        st = skip.mult * st + skip.plus;
     }
 }
 
@@ -132,24 +170,29 @@ __global__ __launch_bounds__(256) void k_gen_zipf(uint8_t* mask, uint64_t n, uin
 
 // ---------------------------------------------------------------- runs
 // pass 1: flips per word -> popcount
-__global__ __launch_bounds__(256) void k_gen_runs_count(uint32_t* counts, uint64_t n, uint32_t thr, uint64_t st0)
+__global__ __launch_bounds__(256) void k_gen_runs_count(uint32_t* counts, uint64_t n, uint32_t thr, uint64_t st0, Affine skip)
 {
     const uint64_t nwords = (n + 31) / 32;
-    for (uint64_t wi = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; wi < nwords; wi += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t st = pcg_advance(st0, wi * 32);
+    const uint64_t wi0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (wi0 >= nwords) return;
+    uint64_t st = pcg_advance(st0, wi0 * 32);
+    for (uint64_t wi = wi0; wi < nwords; wi += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t f = 0;
         for (int k = 0; k < 32; k++) f |= (uint32_t)(pcg_next(st) < thr) << k;
         counts[wi] = popc32(f & valid_bits(wi, n));
+        st = skip.mult * st + skip.plus;
     }
 }
 
 // pass 2: value = parity of all flips up to and including the element
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_gen_runs_write(uint8_t* mask, const uint64_t* before, uint64_t n, uint32_t thr, uint64_t st0)
+__global__ __launch_bounds__(256) void k_gen_runs_write(uint8_t* mask, const uint64_t* before, uint64_t n, uint32_t thr, uint64_t st0, Affine skip)
 {
     const uint64_t nwords = (n + 31) / 32, nbytes = (n + 7) / 8;
-    for (uint64_t wi = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; wi < nwords; wi += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t st = pcg_advance(st0, wi * 32);
+    const uint64_t wi0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (wi0 >= nwords) return;
+    uint64_t st = pcg_advance(st0, wi0 * 32);
+    for (uint64_t wi = wi0; wi < nwords; wi += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t f = 0;
         for (int k = 0; k < 32; k++) f |= (uint32_t)(pcg_next(st) < thr) << k;
         uint32_t x = f;  // inclusive prefix xor along the word
@@ -160,59 +203,86 @@ __global__ __launch_bounds__(256) void k_gen_runs_write(uint8_t* mask, const uin
         x ^= x << 16;
         if (before[wi] & 1ull) x = ~x;
         store_mask_word<ORDER>(mask, wi, x & valid_bits(wi, n), nbytes);
+        st = skip.mult * st + skip.plus;
     }
 }
 
 // ---------------------------------------------------------------- values
-// each thread produces a run of kValueRun consecutive elements after one jump
-constexpr uint64_t kValueRun = 16;
+// A group is 16 bytes of output: kGroup<W> consecutive elements.  Lane t of
+// S lanes makes groups t, t+S, ...: one jump, the group's draws in sequence,
+// then `skip` (the map of the draws of S-1 groups) to its next group, so each
+// draw costs one LCG step and consecutive lanes store consecutive 16 bytes.
+template <int W> constexpr uint32_t kGroup = 16 / W;
+template <int W> constexpr uint32_t kDraws = W == 8 ? 2 : 1;  // draws per element
+
+template <int W> __device__ __forceinline__ uint64_t make_value(uint64_t& st, int kind)
+{
+    if constexpr (W == 8) {
+        const uint64_t hi = pcg_next(st), lo = pcg_next(st);
+        uint64_t v = (hi << 32) | lo;
+        if (kind == CSX_VALUES_FLOAT) v = __double_as_longlong((double)(v >> 11) * (1.0 / 9007199254740992.0));
+        return v;
+    }
+    else {
+        uint32_t v = pcg_next(st);
+        if (W == 4 && kind == CSX_VALUES_FLOAT) v = __float_as_uint((float)(v >> 8) * (1.0f / 16777216.0f));
+        return v;
+    }
+}
 
 template <int W>
-__global__ __launch_bounds__(256) void k_gen_values(uint8_t* out, uint64_t n, int kind, uint64_t st0)
+__global__ __launch_bounds__(256) void k_gen_values(uint8_t* out, uint64_t n, int kind, uint64_t st0, Affine skip, int aligned)
 {
-    const uint64_t runs = (n + kValueRun - 1) / kValueRun;
-    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < runs; r += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t e0 = r * kValueRun;
-        const uint64_t e1 = e0 + kValueRun < n ? e0 + kValueRun : n;
-        uint64_t st = pcg_advance(st0, (W == 8 ? 2 : 1) * e0);
-        for (uint64_t e = e0; e < e1; e++) {
-            if constexpr (W == 8) {
-                const uint64_t hi = pcg_next(st), lo = pcg_next(st);
-                uint64_t v = (hi << 32) | lo;
-                if (kind == 1) {
-                    const double d = (double)(v >> 11) * (1.0 / 9007199254740992.0);
-                    v = __double_as_longlong(d);
-                }
-                reinterpret_cast<uint64_t*>(out)[e] = v;
-            }
-            else {
-                uint32_t v = pcg_next(st);
-                if (W == 4 && kind == 1) v = __float_as_uint((float)(v >> 8) * (1.0f / 16777216.0f));
-                if constexpr (W == 4) reinterpret_cast<uint32_t*>(out)[e] = v;
-                if constexpr (W == 2) reinterpret_cast<uint16_t*>(out)[e] = (uint16_t)v;
-                if constexpr (W == 1) out[e] = (uint8_t)v;
-            }
+    using T = typename word_t<W>::type;
+    constexpr uint32_t G = kGroup<W>;
+    const uint64_t groups = (n + G - 1) / G;
+    const uint64_t g0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (g0 >= groups) return;
+    uint64_t st = pcg_advance(st0, g0 * G * kDraws<W>);
+    for (uint64_t g = g0; g < groups; g += (uint64_t)gridDim.x * blockDim.x) {
+        union {
+            T e[G];
+            u32x4 v;
+        } pack;
+#pragma unroll
+        for (uint32_t j = 0; j < G; j++) pack.e[j] = (T)make_value<W>(st, kind);
+        const uint64_t e0 = g * G;
+        if (aligned && e0 + G <= n) {
+            *reinterpret_cast<u32x4*>(out + e0 * W) = pack.v;
         }
+        else {
+            for (uint32_t j = 0; j < G; j++)
+                if (e0 + j < n) reinterpret_cast<T*>(out)[e0 + j] = pack.e[j];
+        }
+        st = skip.mult * st + skip.plus;
     }
 }
 
 // ---------------------------------------------------------------- launchers
+// 8 blocks of 256 per CU on 256 CUs fill the chip; lanes beyond that loop
 static unsigned gen_grid(uint64_t items)
 {
     uint64_t g = (items + 255) / 256;
     if (g < 1) g = 1;
-    if (g > 16384) g = 16384;
+    if (g > 2048) g = 2048;
     return (unsigned)g;
+}
+
+// map from the end of one item of `draws` draws to the lane's next item
+static Affine lane_skip(unsigned grid, uint64_t draws)
+{
+    return lcg_steps(((uint64_t)grid * 256 - 1) * draws);
 }
 
 hipError_t launch_gen_bernoulli(uint8_t* mask, uint64_t n, uint64_t thr, uint64_t seed, int order, hipStream_t s)
 {
     const uint64_t st0 = splitmix_seed(seed);
     const unsigned g = gen_grid((n + 31) / 32);
+    const Affine skip = lane_skip(g, 32);
     if (order == kBitMsb)
-        k_gen_bernoulli<kBitMsb><<<g, 256, 0, s>>>(mask, n, thr, st0);
+        k_gen_bernoulli<kBitMsb><<<g, 256, 0, s>>>(mask, n, thr, st0, skip);
     else
-        k_gen_bernoulli<kBitLsb><<<g, 256, 0, s>>>(mask, n, thr, st0);
+        k_gen_bernoulli<kBitLsb><<<g, 256, 0, s>>>(mask, n, thr, st0, skip);
     return hipGetLastError();
 }
 
@@ -250,28 +320,32 @@ hipError_t launch_gen_runs(uint8_t* mask, uint64_t n, uint32_t thr, uint64_t see
     const uint64_t st0 = splitmix_seed(seed);
     const uint64_t nwords = (n + 31) / 32;
     const unsigned g = gen_grid(nwords);
-    k_gen_runs_count<<<g, 256, 0, s>>>(counts, n, thr, st0);
+    const Affine skip = lane_skip(g, 32);
+    k_gen_runs_count<<<g, 256, 0, s>>>(counts, n, thr, st0, skip);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = launch_scan(counts, before, nwords, nullptr, hdr, status, s);
     if (e != hipSuccess) return e;
     if (order == kBitMsb)
-        k_gen_runs_write<kBitMsb><<<g, 256, 0, s>>>(mask, before, n, thr, st0);
+        k_gen_runs_write<kBitMsb><<<g, 256, 0, s>>>(mask, before, n, thr, st0, skip);
     else
-        k_gen_runs_write<kBitLsb><<<g, 256, 0, s>>>(mask, before, n, thr, st0);
+        k_gen_runs_write<kBitLsb><<<g, 256, 0, s>>>(mask, before, n, thr, st0, skip);
     return hipGetLastError();
 }
 
 hipError_t launch_gen_values(void* out, uint64_t n, uint32_t w, int kind, uint64_t seed, hipStream_t s)
 {
     const uint64_t st0 = splitmix_seed(seed);
-    const unsigned g = gen_grid((n + kValueRun - 1) / kValueRun);
     auto o = static_cast<uint8_t*>(out);
+    const int aligned = ((uintptr_t)out & 15) == 0;
+    const uint64_t groups = (n * w + 15) / 16;
+    const unsigned g = gen_grid(groups);
+    const Affine skip = lane_skip(g, (16 / w) * (w == 8 ? 2 : 1));  // draws per group: 16/w elements, 2 draws each at w = 8
     switch (w) {
-        case 1: k_gen_values<1><<<g, 256, 0, s>>>(o, n, kind, st0); break;
-        case 2: k_gen_values<2><<<g, 256, 0, s>>>(o, n, kind, st0); break;
-        case 4: k_gen_values<4><<<g, 256, 0, s>>>(o, n, kind, st0); break;
-        default: k_gen_values<8><<<g, 256, 0, s>>>(o, n, kind, st0); break;
+        case 1: k_gen_values<1><<<g, 256, 0, s>>>(o, n, kind, st0, skip, aligned); break;
+        case 2: k_gen_values<2><<<g, 256, 0, s>>>(o, n, kind, st0, skip, aligned); break;
+        case 4: k_gen_values<4><<<g, 256, 0, s>>>(o, n, kind, st0, skip, aligned); break;
+        default: k_gen_values<8><<<g, 256, 0, s>>>(o, n, kind, st0, skip, aligned); break;
     }
     return hipGetLastError();
 }

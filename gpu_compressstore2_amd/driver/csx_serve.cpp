@@ -72,6 +72,20 @@ struct Live {
     ReadyEvent* ready;
 };
 
+// One asynchronous check of a finished request: csx_compact of the same
+// request alone into v_out, then csx_validate against the served output.
+// The request's buffers stay allocated until the check's event has fired.
+struct Check {
+    void* v_out = nullptr;
+    void* ws = nullptr;
+    uint64_t* d_res = nullptr;  // [0] count alone, [1] differing elements
+    uint64_t* h_res = nullptr;  // pinned copy of d_res
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    Live req{};
+    uint64_t served_count = 0;
+};
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -107,15 +121,18 @@ int main(int argc, char** argv)
     hipStream_t gen, ver;
     hk(hipStreamCreate(&gen), "stream");
     hk(hipStreamCreate(&ver), "stream");
-    // verification buffers: the largest request alone
+    // verification contexts, each sized for the largest request alone
     const uint64_t nmax = 1ull << lmax;
-    void *v_out = nullptr, *v_ws = nullptr;
-    uint64_t* v_fail = nullptr;
     size_t v_ws_bytes = 0;
     ck(csx_workspace_size(nmax, 8, 1024, &v_ws_bytes), "workspace size");
-    hk(hipMalloc(&v_out, nmax * 8), "hipMalloc");
-    hk(hipMalloc(&v_ws, v_ws_bytes), "hipMalloc");
-    hk(hipMalloc(&v_fail, sizeof(uint64_t)), "hipMalloc");
+    std::vector<Check> checks(4);
+    for (Check& c : checks) {
+        hk(hipMalloc(&c.v_out, nmax * 8), "hipMalloc");
+        hk(hipMalloc(&c.ws, v_ws_bytes), "hipMalloc");
+        hk(hipMalloc((void**)&c.d_res, 2 * sizeof(uint64_t)), "hipMalloc");
+        hk(hipHostMalloc((void**)&c.h_res, 2 * sizeof(uint64_t), hipHostMallocDefault), "hipHostMalloc");
+        hk(hipEventCreateWithFlags(&c.done, hipEventDisableTiming), "event");
+    }
 
     std::mt19937_64 rng(seed);
     std::exponential_distribution<double> gap(rate / 1000.0);  // per ms
@@ -127,6 +144,52 @@ int main(int argc, char** argv)
     uint64_t next_id = 0, admitted = 0, completed = 0, verified = 0, mismatches = 0, waits = 0;
     double bytes = 0.0, last_done_ms = 0.0;
     uint64_t peak_used = 0;
+    double host_admit = 0, host_step = 0, host_poll = 0, host_check = 0;
+    auto release = [&](Live& L) {
+        csx_arena_free(arena, L.in);
+        csx_arena_free(arena, L.mask);
+        csx_arena_free(arena, L.out);
+        if (--L.ready->refs == 0) {
+            hk(hipEventDestroy(L.ready->ev), "event destroy");
+            delete L.ready;
+        }
+    };
+    auto retire = [&](Check& c) {
+        verified++;
+        if (c.h_res[0] != c.served_count || c.h_res[1] != 0) mismatches++;
+        release(c.req);
+        c.busy = false;
+    };
+    // retire finished checks; with wait, block for the oldest busy one
+    auto sweep_checks = [&](bool wait) {
+        for (Check& c : checks) {
+            if (!c.busy) continue;
+            if (wait) hk(hipEventSynchronize(c.done), "check wait");
+            const hipError_t q = hipEventQuery(c.done);
+            if (q == hipSuccess) retire(c);
+            else if (q != hipErrorNotReady) hk(q, "check query");
+        }
+    };
+    auto launch_check = [&](Live& L, uint64_t served) {
+        Check* c = nullptr;
+        for (Check& k : checks)
+            if (!k.busy) c = &k;
+        if (!c) {
+            sweep_checks(true);
+            for (Check& k : checks)
+                if (!k.busy) c = &k;
+        }
+        c->busy = true;
+        c->req = L;
+        c->served_count = served;
+        hk(hipMemsetAsync(c->d_res, 0, 2 * sizeof(uint64_t), ver), "memset");
+        ck(csx_compact(L.in, static_cast<const uint8_t*>(L.mask), c->v_out, L.spec.n, L.spec.w, 1024, CSX_MSB_FIRST, CSX_PIPELINE_THREE_PHASE,
+                       c->d_res, nullptr, c->ws, v_ws_bytes, ver),
+           "check compact");
+        ck(csx_validate(c->v_out, L.out, served, L.spec.w, c->d_res + 1, ver), "check validate");
+        hk(hipMemcpyAsync(c->h_res, c->d_res, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, ver), "check copy");
+        hk(hipEventRecord(c->done, ver), "check record");
+    };
     const double t0 = now_ms();
     double next_arrival = t0 + gap(rng);
     const double t_end = t0 + seconds * 1000.0;
@@ -147,6 +210,7 @@ int main(int argc, char** argv)
             next_arrival += gap(rng);
         }
         // admission in arrival order while the budget allows
+        const double ta = now_ms();
         ReadyEvent* round = nullptr;
         std::vector<uint64_t> fresh;
         while (!arrivals.empty()) {
@@ -195,10 +259,15 @@ int main(int argc, char** argv)
         }
         if (!arrivals.empty()) waits++;
         peak_used = std::max(peak_used, csx_arena_used(arena));
+        const double ts = now_ms();
+        host_admit += ts - ta;
         uint32_t launched = 0, nd = 0;
         ck(csx_server_step(server, &launched), "step");
+        const double tq = now_ms();
+        host_step += tq - ts;
         ck(csx_server_poll(server, done.data(), (uint32_t)done.size(), &nd), "poll");
         const double tp = now_ms();
+        host_poll += tp - tq;
         for (uint32_t i = 0; i < nd; i++) {
             auto it = live.find(done[i].id);
             if (it == live.end()) die("poll", "unknown request id");
@@ -207,29 +276,17 @@ int main(int argc, char** argv)
             bytes += (double)L.spec.n * L.spec.w + (double)((L.spec.n + 7) / 8) + (double)done[i].count * L.spec.w;
             completed++;
             last_done_ms = tp;
-            if (verify_every > 0 && L.spec.id % (uint64_t)verify_every == 0) {
-                uint64_t k_alone = 0;
-                ck(csx_compact(L.in, static_cast<const uint8_t*>(L.mask), v_out, L.spec.n, L.spec.w, 1024, CSX_MSB_FIRST, CSX_PIPELINE_THREE_PHASE,
-                               nullptr, &k_alone, v_ws, v_ws_bytes, ver),
-                   "verify compact");
-                uint64_t fails = 0;
-                hk(hipMemsetAsync(v_fail, 0, sizeof(uint64_t), ver), "memset");
-                ck(csx_validate(v_out, L.out, k_alone, L.spec.w, v_fail, ver), "validate");
-                hk(hipMemcpyAsync(&fails, v_fail, sizeof(uint64_t), hipMemcpyDeviceToHost, ver), "copy");
-                hk(hipStreamSynchronize(ver), "sync");
-                verified++;
-                if (fails || k_alone != done[i].count) mismatches++;
-            }
-            csx_arena_free(arena, L.in);
-            csx_arena_free(arena, L.mask);
-            csx_arena_free(arena, L.out);
-            if (--L.ready->refs == 0) {
-                hk(hipEventDestroy(L.ready->ev), "event destroy");
-                delete L.ready;
-            }
+            if (verify_every > 0 && L.spec.id % (uint64_t)verify_every == 0) launch_check(L, done[i].count);
+            else release(L);
             live.erase(it);
         }
-        if (tp >= t_end && arrivals.empty() && live.empty()) break;
+        sweep_checks(false);
+        const double tc = now_ms();
+        host_check += tc - tp;
+        if (tp >= t_end && arrivals.empty() && live.empty()) {
+            sweep_checks(true);
+            break;
+        }
         if (nd == 0 && launched == 0 && arrivals.empty() && next_arrival > tp + 0.05 && tp < t_end) usleep(20);
     }
     const double wall_ms = last_done_ms - t0;
@@ -240,15 +297,21 @@ int main(int argc, char** argv)
     mean = latencies.empty() ? 0 : mean / latencies.size();
     printf("{\"requests\": %llu, \"completed\": %llu, \"seconds\": %.3f, \"offered_rate\": %.0f, \"achieved_rate\": %.1f, \"GB/s\": %.1f, "
            "\"p50_ms\": %.3f, \"p99_ms\": %.3f, \"max_ms\": %.3f, \"mean_ms\": %.3f, \"verified\": %llu, \"mismatches\": %llu, "
-           "\"peak_budget_GiB\": %.2f, \"budget_GiB\": %.1f, \"admission_waits\": %llu, \"streams\": %d, \"max_batch\": %d}\n",
+           "\"peak_budget_GiB\": %.2f, \"budget_GiB\": %.1f, \"admission_waits\": %llu, \"streams\": %d, \"max_batch\": %d, "
+           "\"host_ms\": {\"admit\": %.0f, \"step\": %.0f, \"poll\": %.0f, \"check\": %.0f}}\n",
            (unsigned long long)next_id, (unsigned long long)completed, wall_ms / 1000.0, rate, completed / (wall_ms / 1000.0),
            bytes / (wall_ms * 1e-3) / 1e9, pct(0.50), pct(0.99), latencies.empty() ? 0.0 : latencies.back(), mean, (unsigned long long)verified,
-           (unsigned long long)mismatches, peak_used / double(1ull << 30), budget_gib, (unsigned long long)waits, streams, max_batch);
+           (unsigned long long)mismatches, peak_used / double(1ull << 30), budget_gib, (unsigned long long)waits, streams, max_batch, host_admit,
+           host_step, host_poll, host_check);
     ck(csx_server_destroy(server), "server destroy");
     ck(csx_arena_destroy(arena), "arena destroy");
-    (void)hipFree(v_out);
-    (void)hipFree(v_ws);
-    (void)hipFree(v_fail);
+    for (Check& c : checks) {
+        (void)hipFree(c.v_out);
+        (void)hipFree(c.ws);
+        (void)hipFree(c.d_res);
+        (void)hipHostFree(c.h_res);
+        (void)hipEventDestroy(c.done);
+    }
     (void)hipStreamDestroy(gen);
     (void)hipStreamDestroy(ver);
     return mismatches ? 1 : 0;
