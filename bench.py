@@ -17,6 +17,7 @@ own stream; "cpu_baseline" times the reference's CPU compaction
 (generate_validation, compiled from its sources into oracle/_ref) on the host.
 """
 import argparse
+import subprocess
 import json
 import os
 import platform
@@ -181,6 +182,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline")
     ap.add_argument("--sels", default="0.01,0.5,0.99", help="selectivities to sweep (0.5 is always included)")
     ap.add_argument("--pipelines", default="three_phase,two_pass,fused", help="pipelines to sweep (three_phase is always included)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the serving-loop run (0 skips it)")
     args = ap.parse_args()
     sels = sorted(set(float(x) for x in args.sels.split(",")) | {0.5})
     all_pipes = {"three_phase": csx.PIPELINE_THREE_PHASE, "two_pass": csx.PIPELINE_TWO_PASS, "fused": csx.PIPELINE_FUSED}
@@ -273,6 +275,9 @@ def main():
         sweep["batch_4096_columns"] = bench_batch(args.steps, args.warmup)
         datas = [make_data(n, 100 + r) for r in range(ROTATE)]
 
+    if args.serve_seconds > 0:
+        sweep["serving_poisson_20k"] = bench_serving(args.serve_seconds)
+
     # plain device copy of the input size (csx_copy): what this HBM sustains
     copy_ms = time_steps(lambda i: csx.copy(out, datas[i % ROTATE]), K, W)
     sweep["copy_4N"] = {"GB/s": round(K * 2 * 4 * n / (copy_ms * 1e-3) / 1e9, 1), "ms_per_step": round(copy_ms / K, 5)}
@@ -327,6 +332,23 @@ def main():
         "sweep": sweep,
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_serving(seconds):
+    """The serving loop (gpu_compressstore2_amd/csx_serve, its own process):
+    Poisson arrivals at 20k requests/s of u32/f32/u64 requests, lengths
+    log-uniform in [2^12, 2^26], Bernoulli masks with p uniform in [0, 1],
+    64 GiB device budget; every 64th request is recomputed alone and compared."""
+    exe = os.path.join(ROOT, "gpu_compressstore2_amd", "csx_serve")
+    print(f"[bench] serving loop, {seconds:g} s at 20000 requests/s", file=sys.stderr, flush=True)
+    r = subprocess.run([exe, "-t", str(seconds)], capture_output=True, text=True, timeout=seconds + 240)
+    if r.returncode != 0:
+        print(f"[bench] serving loop failed ({r.returncode}): {r.stderr[-2000:]}", file=sys.stderr)
+        sys.exit(1)
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    print(f"[bench] serving: {res['achieved_rate']:.0f} req/s, {res['GB/s']} GB/s, p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms", file=sys.stderr,
+          flush=True)
+    return res
 
 
 def cpu_baseline(data_dev, n, budget_s):
