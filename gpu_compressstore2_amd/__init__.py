@@ -102,6 +102,9 @@ SIGNATURES = {
     "csx_pattern_compact": (_int, [_vp, _vp, _u64, _u32, _u32, _u32, _vp, ctypes.POINTER(_u64), _vp]),
     "csx_profile_begin": (_int, []),
     "csx_profile_end": (_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
+    "csx_predicate_mask": (_int, [_vp, _u64, _vp, _vp, _int, _vp, _vp]),
+    "csx_predicate_workspace_size": (_int, [_u64, _int, ctypes.POINTER(_sz)]),
+    "csx_compact_predicate": (_int, [_vp, _u64, _vp, _vp, _vp, _int, _vp, ctypes.POINTER(_u64), _vp, _sz, _vp]),
     # serving: struct arguments travel as void* (structs in serving.py)
     "csx_arena_create": (_int, [_u64, ctypes.POINTER(_vp)]),
     "csx_arena_destroy": (_int, [_vp]),
@@ -284,6 +287,80 @@ def compact(inp, mask, n=None, chunk_len=1024, bit_order=MSB_FIRST, pipeline=PIP
         lib().csx_compact(
             _ptr(inp), _ptr(mask), _ptr(out), n, w, chunk_len, bit_order, pipeline, _ptr(count),
             ctypes.byref(h) if host_count else None, _ptr(workspace), workspace.numel(), _stream(stream),
+        )
+    )
+    return out, (int(h.value) if host_count else count)
+
+
+# ---------------------------------------------------------------- predicate
+PRED_GT, PRED_GE, PRED_LT, PRED_LE, PRED_EQ, PRED_NE, PRED_BETWEEN = range(7)
+PRED_OPS = {">": PRED_GT, ">=": PRED_GE, "<": PRED_LT, "<=": PRED_LE, "==": PRED_EQ, "!=": PRED_NE, "between": PRED_BETWEEN}
+# csx_type index -> numpy dtype name; torch dtype -> csx_type index
+TYPE_NAMES = ["uint8", "uint16", "uint32", "uint64", "int8", "int16", "int32", "int64", "float32", "float64"]
+_TORCH_TYPES = {
+    torch.uint8: 0, torch.int8: 4, torch.int16: 5, torch.int32: 6, torch.int64: 7, torch.float32: 8, torch.float64: 9,
+    getattr(torch, "uint16", None): 1, getattr(torch, "uint32", None): 2, getattr(torch, "uint64", None): 3,
+}
+
+
+class Predicate(ctypes.Structure):
+    """csx_predicate: the csx_scalar unions travel as their 8 raw bytes."""
+
+    _fields_ = [("type", _int), ("op", _int), ("a", _u64), ("b", _u64)]
+
+
+def predicate(col_type, op, a, b=0):
+    """csx_predicate for a column of csx_type index `col_type` (or a torch dtype)."""
+    import numpy as np
+
+    if not isinstance(col_type, int):
+        col_type = _TORCH_TYPES[col_type]
+    op = PRED_OPS.get(op, op)
+
+    def bits(v):
+        raw = np.array([v], dtype=TYPE_NAMES[col_type]).view(np.uint8)
+        return int.from_bytes(bytes(raw) + bytes(8 - raw.size), "little")
+
+    return Predicate(col_type, op, bits(a), bits(b))
+
+
+def predicate_mask(col, op, a, b=0, n=None, col_type=None, bit_order=MSB_FIRST, mask=None, count=None, stream=None):
+    """Packed mask of `col OP a` (or a <= col <= b).  Returns (mask, count tensor)."""
+    _require_device(col, mask, count)
+    n = col.numel() if n is None else n
+    q = predicate(col.dtype if col_type is None else col_type, op, a, b)
+    if mask is None:
+        mask = torch.empty(max((n + 7) // 8, 1), dtype=torch.uint8, device=col.device)
+    if count is None:
+        count = torch.empty(1, dtype=torch.int64, device=col.device)
+    _check(lib().csx_predicate_mask(_ptr(col), n, ctypes.byref(q), _ptr(mask), bit_order, _ptr(count), _stream(stream)))
+    return mask, count
+
+
+def predicate_workspace_size(n, col_type):
+    b = _sz(0)
+    _check(lib().csx_predicate_workspace_size(n, col_type, ctypes.byref(b)))
+    return b.value
+
+
+def compact_predicate(col, op, a, b=0, n=None, col_type=None, bit_order=MSB_FIRST, out=None, mask=None, count=None, workspace=None, stream=None,
+                      host_count=True):
+    """Fused predicate + compaction: (out, k); the mask is written to `mask` when given."""
+    _require_device(col, out, mask, count)
+    n = col.numel() if n is None else n
+    t = _TORCH_TYPES[col.dtype] if col_type is None else col_type
+    q = predicate(t, op, a, b)
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=col.dtype, device=col.device)
+    if count is None:
+        count = torch.empty(1, dtype=torch.int64, device=col.device)
+    if workspace is None:
+        workspace = alloc_workspace(predicate_workspace_size(n, t), col.device)
+    h = _u64(0)
+    _check(
+        lib().csx_compact_predicate(
+            _ptr(col), n, ctypes.byref(q), _ptr(out), _ptr(mask), bit_order, _ptr(count), ctypes.byref(h) if host_count else None,
+            _ptr(workspace), workspace.numel(), _stream(stream),
         )
     )
     return out, (int(h.value) if host_count else count)

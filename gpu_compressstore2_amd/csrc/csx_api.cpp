@@ -209,6 +209,43 @@ template <typename F> hipError_t timed(hipStream_t s, F&& fn)
     return hipEventRecord(b, s);
 }
 
+// csx_predicate -> one range test; returns the element size (0 if invalid)
+uint32_t pred_spec(const csx_predicate* q, csx::PredSpec* p)
+{
+    static const uint32_t sizes[] = {1, 2, 4, 8, 1, 2, 4, 8, 4, 8};
+    if (q->type < CSX_TYPE_U8 || q->type > CSX_TYPE_F64) return 0;
+    p->type = q->type;
+    p->lo = p->hi = q->a.u64;
+    using namespace csx;
+    switch (q->op) {
+        case CSX_PRED_GT: p->flags = kPredLo; break;
+        case CSX_PRED_GE: p->flags = kPredLo | kPredLoIncl; break;
+        case CSX_PRED_LT: p->flags = kPredHi; break;
+        case CSX_PRED_LE: p->flags = kPredHi | kPredHiIncl; break;
+        case CSX_PRED_EQ: p->flags = kPredLo | kPredLoIncl | kPredHi | kPredHiIncl; break;
+        case CSX_PRED_NE: p->flags = kPredLo | kPredLoIncl | kPredHi | kPredHiIncl | kPredNegate; break;
+        case CSX_PRED_BETWEEN:
+            p->flags = kPredLo | kPredLoIncl | kPredHi | kPredHiIncl;
+            p->hi = q->b.u64;
+            break;
+        default: return 0;
+    }
+    return sizes[q->type];
+}
+
+csx_status check_pred(const csx_predicate* q, csx::PredSpec* p, uint32_t* w)
+{
+    if (!q) return fail(CSX_ERR_INVALID, "null predicate");
+    if (q->type < CSX_TYPE_U8 || q->type > CSX_TYPE_F64) return fail(CSX_ERR_INVALID, "unknown element type %d", q->type);
+    if (!(*w = pred_spec(q, p))) return fail(CSX_ERR_INVALID, "unknown comparison %d", q->op);
+    return CSX_OK;
+}
+
+// predicate compaction workspace: the csx_compact plan at chunk 1024 | mask
+constexpr uint32_t kPredChunk = 1024;
+size_t pred_mask_off(uint64_t n, uint32_t w) { return align_up(plan_compact(n, w, kPredChunk).total); }
+size_t pred_ws_total(uint64_t n, uint32_t w) { return pred_mask_off(n, w) + align_up((n + 7) / 8); }
+
 } // namespace
 
 extern "C" {
@@ -865,6 +902,78 @@ csx_status csx_profile_end(double* total_ms, uint64_t* launches)
     if (total_ms) *total_ms = sum;
     if (launches) *launches = count;
     if (err != hipSuccess) return hip_fail(err, "csx_profile_end");
+    return ok();
+}
+
+csx_status csx_predicate_mask(const void* d_in, uint64_t n, const csx_predicate* pred, uint8_t* d_mask, int bit_order, uint64_t* d_count,
+                              hipStream_t stream)
+{
+    csx_status s;
+    csx::PredSpec p;
+    uint32_t w = 0;
+    if ((s = check_pred(pred, &p, &w)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
+    if (n > 0 && (!d_in || !d_mask)) return fail(CSX_ERR_INVALID, "null input or mask pointer");
+    if ((uintptr_t)d_in % w) return fail(CSX_ERR_INVALID, "input must be aligned to the element size");
+    if (d_count) CSX_HIP(csx::launch_store_u64(d_count, 0, stream), "csx_predicate_mask count");
+    if (n == 0) return ok();
+    CSX_HIP(csx::launch_predicate(d_in, n, p, d_mask, bit_order, nullptr, d_count, nullptr, nullptr, 0, stream), "csx_predicate_mask");
+    return ok();
+}
+
+csx_status csx_predicate_workspace_size(uint64_t n, int type, size_t* bytes)
+{
+    csx_status s;
+    if ((s = check_n(n))) return s;
+    if (!bytes) return fail(CSX_ERR_INVALID, "null result pointer");
+    csx_predicate q{};
+    q.type = type;
+    csx::PredSpec p;
+    uint32_t w = 0;
+    if ((s = check_pred(&q, &p, &w))) return s;
+    *bytes = pred_ws_total(n, w);
+    return ok();
+}
+
+csx_status csx_compact_predicate(const void* d_in, uint64_t n, const csx_predicate* pred, void* d_out, uint8_t* d_mask, int bit_order,
+                                 uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream)
+{
+    csx_status s;
+    csx::PredSpec p;
+    uint32_t w = 0;
+    if ((s = check_pred(pred, &p, &w)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
+    if ((s = check_ws(d_ws, ws_bytes, pred_ws_total(n, w)))) return s;
+    if (n > 0 && (!d_in || !d_out)) return fail(CSX_ERR_INVALID, "null input or output pointer");
+    if (((uintptr_t)d_in % w) || ((uintptr_t)d_out % w)) return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
+    const CompactPlan cp = plan_compact(n, w, kPredChunk);
+    if (!launch_fits(cp.fused_tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)cp.fused_tiles);
+
+    auto base = static_cast<uint8_t*>(d_ws);
+    auto hdr = reinterpret_cast<csx::WsHeader*>(base);
+    auto status = reinterpret_cast<uint64_t*>(base + cp.status_off);
+    auto counts = reinterpret_cast<uint32_t*>(base + cp.counts_off);
+    auto offsets = reinterpret_cast<uint64_t*>(base + cp.offsets_off);
+    uint8_t* mask = d_mask ? d_mask : base + pred_mask_off(n, w);
+    uint64_t* count_dst = d_count ? d_count : &hdr->count;
+    if (n == 0) {
+        CSX_HIP(csx::launch_lookback_init(hdr, status, 0, stream), "csx_compact_predicate init");
+        CSX_HIP(csx::launch_store_u64(count_dst, 0, stream), "csx_compact_predicate count");
+    }
+    else {
+        CSX_HIP(csx::launch_predicate(d_in, n, p, mask, bit_order, counts, nullptr, hdr, status, cp.scan_tiles, stream), "csx_compact_predicate mask");
+        CSX_HIP(csx::launch_scan(counts, offsets, cp.chunks, count_dst, hdr, status, stream, true), "csx_compact_predicate scan");
+        csx::TileArgs a{};
+        a.in = static_cast<const uint8_t*>(d_in);
+        a.mask = mask;
+        a.out = static_cast<uint8_t*>(d_out);
+        a.n = n;
+        a.offsets = offsets;
+        a.counts = counts;
+        a.log2c = log2u(kPredChunk);
+        const bool aligned = (((uintptr_t)d_in | (uintptr_t)mask) & 15) == 0;
+        CSX_HIP(timed(stream, [&] { return dispatch_tile(w, csx::kModeWriteout, true, bit_order, aligned, a, stream); }),
+                "csx_compact_predicate writeout");
+    }
+    if (h_count) return finish_host_count(h_count, count_dst, hdr, stream);
     return ok();
 }
 

@@ -188,6 +188,21 @@ __device__ __forceinline__ uint64_t lookback_exclusive(const uint64_t* status, u
 
 // ---- misc ---------------------------------------------------------------
 
+// store 32 mask bits (LSB-first word of elements 32*wi ..) in the caller's order,
+// without touching bytes at or past nbytes
+template <int ORDER> __device__ __forceinline__ void store_mask_word(uint8_t* mask, uint64_t wi, uint32_t word, uint64_t nbytes)
+{
+    const uint32_t v = to_lsb_word<ORDER>(word);  // the byte bit-reversal is its own inverse
+    const uint64_t off = wi * 4;
+    if (off + 4 <= nbytes && ((uintptr_t)(mask + off) & 3) == 0) {
+        *reinterpret_cast<uint32_t*>(mask + off) = v;
+    }
+    else {
+        for (int b = 0; b < 4; b++)
+            if (off + b < nbytes) mask[off + b] = (uint8_t)(v >> (8 * b));
+    }
+}
+
 template <int W> struct word_t;
 template <> struct word_t<1> { using type = uint8_t; };
 template <> struct word_t<2> { using type = uint16_t; };

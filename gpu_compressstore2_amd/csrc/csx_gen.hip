@@ -85,21 +85,6 @@ __device__ inline uint64_t pcg_advance(uint64_t st, uint64_t delta)
     return st;
 }
 
-// store 32 mask bits (LSB-first word of elements 32*wi ..) in the caller's order,
-// without touching bytes at or past nbytes
-template <int ORDER> __device__ __forceinline__ void store_mask_word(uint8_t* mask, uint64_t wi, uint32_t word, uint64_t nbytes)
-{
-    const uint32_t v = to_lsb_word<ORDER>(word);  // the byte bit-reversal is its own inverse
-    const uint64_t off = wi * 4;
-    if (off + 4 <= nbytes && ((uintptr_t)(mask + off) & 3) == 0) {
-        *reinterpret_cast<uint32_t*>(mask + off) = v;
-    }
-    else {
-        for (int b = 0; b < 4; b++)
-            if (off + b < nbytes) mask[off + b] = (uint8_t)(v >> (8 * b));
-    }
-}
-
 __device__ __forceinline__ uint32_t valid_bits(uint64_t wi, uint64_t n)
 {
     const uint64_t b = wi * 32;

@@ -37,6 +37,10 @@ hipError_t launch_gen_zipf(uint8_t* mask, uint64_t n, uint32_t block_len, uint64
 hipError_t launch_gen_runs(uint8_t* mask, uint64_t n, uint32_t thr, uint64_t seed, int order, uint32_t* counts, uint64_t* before, WsHeader* hdr,
                            uint64_t* status, hipStream_t s);
 hipError_t launch_gen_values(void* out, uint64_t n, uint32_t w, int kind, uint64_t seed, hipStream_t s);
+// column + comparison -> mask (csx_predicate.hip); counts per 1024-element
+// chunk and the running total are optional, hdr/status as in launch_popc
+hipError_t launch_predicate(const void* in, uint64_t n, const PredSpec& p, uint8_t* mask, int order, uint32_t* counts, uint64_t* total, WsHeader* hdr,
+                            uint64_t* status, uint64_t init_tiles, hipStream_t s);
 // periodic-pattern compaction (csx_pattern.hip); k = selected count
 hipError_t launch_pattern(uint32_t w, const void* in, void* out, uint64_t k, uint32_t pattern, uint32_t L, hipStream_t s);
 hipError_t launch_validate(const void* a, const void* b, uint64_t len, uint32_t w, uint64_t* failures, hipStream_t s);

@@ -57,6 +57,16 @@ struct Segment {
     uint64_t n;
 };
 
+// A predicate as one range test over the element type (csx_predicate.hip):
+// selected = (!(flags & kPredLo) || v >(=) lo) && (!(flags & kPredHi) || v <(=) hi),
+// xor (flags & kPredNegate); lo/hi hold the bounds' bytes (little-endian low bytes).
+constexpr uint32_t kPredLo = 1, kPredLoIncl = 2, kPredHi = 4, kPredHiIncl = 8, kPredNegate = 16;
+struct PredSpec {
+    int type;  // csx_type
+    uint64_t lo, hi;
+    uint32_t flags;
+};
+
 // Tiles per group of the two-pass pipeline: a power of two >= 16 keeping
 // the group count at most 2048 (the writeout sums at most 8 group sums per
 // thread), capped at 2048 tiles per group (the pass-1 LDS table).

@@ -263,6 +263,64 @@ csx_status csx_server_poll(csx_server* server, csx_completion* done, uint32_t ca
 uint32_t csx_server_queued(const csx_server* server);
 uint32_t csx_server_inflight(const csx_server* server);
 
+/* ---------------------------------------------------------------- predicate
+ * The step before the path: a column and a comparison give the packed mask.
+ * csx_predicate_mask replaces gen_predicate (src/utils.cuh:102-122) with the
+ * driver's predicate_function `value > threshold` (src/main.cu:24-28), on the
+ * device, for any element type and comparison; element i is selected iff
+ * `in[i] OP a` (BETWEEN: a <= in[i] <= b), compared in the column's own type
+ * (IEEE: every comparison with NaN is false except NE).  Mask bytes past
+ * ceil(n/8) are not written; bits at index >= n are zero.  d_count (nullable)
+ * receives the number selected.
+ * csx_compact_predicate fuses the predicate into the compaction: one kernel
+ * evaluates it, writes the mask and the per-chunk counts (the popcount phase),
+ * then the look-back scan and the staged writeout run as in csx_compact.
+ * d_mask may be null (the mask then lives in the workspace).  No counterpart
+ * in the reference beyond the two steps it replaces. */
+typedef enum {
+    CSX_TYPE_U8 = 0,
+    CSX_TYPE_U16 = 1,
+    CSX_TYPE_U32 = 2,
+    CSX_TYPE_U64 = 3,
+    CSX_TYPE_I8 = 4,
+    CSX_TYPE_I16 = 5,
+    CSX_TYPE_I32 = 6,
+    CSX_TYPE_I64 = 7,
+    CSX_TYPE_F32 = 8,
+    CSX_TYPE_F64 = 9
+} csx_type;
+typedef enum {
+    CSX_PRED_GT = 0,
+    CSX_PRED_GE = 1,
+    CSX_PRED_LT = 2,
+    CSX_PRED_LE = 3,
+    CSX_PRED_EQ = 4,
+    CSX_PRED_NE = 5,
+    CSX_PRED_BETWEEN = 6
+} csx_pred_op;
+typedef union {
+    uint64_t u64;
+    int64_t i64;
+    double f64;
+    float f32;
+    uint32_t u32;
+    int32_t i32;
+    uint16_t u16;
+    int16_t i16;
+    uint8_t u8;
+    int8_t i8;
+} csx_scalar; /* the member of the column's type is read */
+typedef struct {
+    int type; /* csx_type */
+    int op;   /* csx_pred_op */
+    csx_scalar a, b;
+} csx_predicate;
+csx_status csx_predicate_mask(const void* d_in, uint64_t n, const csx_predicate* pred, uint8_t* d_mask, int bit_order, uint64_t* d_count,
+                              hipStream_t stream);
+csx_status csx_predicate_workspace_size(uint64_t n, int type, size_t* bytes);
+csx_status csx_compact_predicate(const void* d_in, uint64_t n, const csx_predicate* pred, void* d_out, uint8_t* d_mask, int bit_order,
+                                 uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream);
+
 /* Non-zero if a single-pass kernel that used this workspace gave up waiting
  * (never expected; reported as CSX_ERR_DEVICE by synchronous calls).
  * Synchronises the stream. */

@@ -76,6 +76,7 @@ class Oracle:
             "csxo_fill_values": (None, [_vp, _u64, _u32, ctypes.c_int, _u64]),
             "csxo_predicate_mask_f32": (_u64, [_vp, _u64, ctypes.c_float, _vp, ctypes.c_int]),
             "csxo_predicate_mask_u32": (_u64, [_vp, _u64, _u32, _vp, ctypes.c_int]),
+            "csxo_predicate_mask_typed": (_u64, [_vp, _u64, ctypes.c_int, ctypes.c_int, _u64, _u64, _vp, ctypes.c_int]),
             "csxo_generate_validation": (_u64, [_vp, _vp, _vp, _u64, _u32]),
             "csxo_compact": (_u64, [_vp, _vp, _vp, _u64, _u32, ctypes.c_int]),
             "csxo_popc": (None, [_vp, _vp, _u64, _u32, ctypes.c_int]),
@@ -175,6 +176,17 @@ class Oracle:
             ones = self.lib.csxo_predicate_mask_u32(_ptr(col), n, int(threshold), _ptr(mask), bit_order)
         else:
             raise TypeError(col.dtype)
+        return mask, int(ones)
+
+    def predicate_mask_typed(self, col, op, a, b=0, bit_order=MSB_FIRST):
+        """Mask of `col OP a` (op index as csx_pred_op; 6 = a <= v <= b)."""
+        col = np.ascontiguousarray(col)
+        n = col.size
+        mask = np.zeros((n + 7) // 8, np.uint8)
+        ones = self.lib.csxo_predicate_mask_typed(_ptr(col), n, PRED_TYPES[col.dtype], op, scalar_bits(a, col.dtype), scalar_bits(b, col.dtype),
+                                                  _ptr(mask), bit_order)
+        if ones == 2**64 - 1:
+            raise ValueError(f"bad predicate op {op}")
         return mask, int(ones)
 
     # ---- compaction path --------------------------------------------------
@@ -285,6 +297,17 @@ class Reference:
         m = np.array(mask, np.uint8, copy=True)
         cnt = self.lib.ref_generate_validation(_ptr(data), _ptr(m), _ptr(out), n, data.dtype.itemsize)
         return out, int(cnt)
+
+
+# numpy dtype -> csx_type index
+PRED_TYPES = {np.dtype(t): i for i, t in enumerate([np.uint8, np.uint16, np.uint32, np.uint64, np.int8, np.int16, np.int32, np.int64,
+                                                    np.float32, np.float64])}
+
+
+def scalar_bits(v, dtype):
+    """A value of `dtype` as the little-endian low bytes of a uint64."""
+    raw = np.array([v], dtype=dtype).view(np.uint8)
+    return int.from_bytes(bytes(raw) + bytes(8 - raw.size), "little")
 
 
 def reference_available():

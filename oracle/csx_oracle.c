@@ -347,6 +347,57 @@ uint64_t csxo_predicate_mask_u32(const uint32_t* col, uint64_t n, uint32_t thres
     return ones;
 }
 
+/* Any column type and comparison (the device predicate of csx_predicate_mask):
+ * element i is selected iff col[i] OP a, or a <= col[i] <= b for BETWEEN, with
+ * C's own comparison in the column's type.  type: 0..3 u8..u64, 4..7 i8..i64,
+ * 8 f32, 9 f64; op: 0 >, 1 >=, 2 <, 3 <=, 4 ==, 5 !=, 6 between.  a and b are
+ * the little-endian low bytes of the bounds.  Returns the number selected, or
+ * UINT64_MAX for an unknown type or op. */
+#define CSXO_PRED_TYPED(T)                                                      \
+    do {                                                                        \
+        T a, b;                                                                 \
+        memcpy(&a, &a_bits, sizeof(T));                                         \
+        memcpy(&b, &b_bits, sizeof(T));                                         \
+        const T* c = (const T*)col;                                             \
+        for (uint64_t i = 0; i < n; i++) {                                      \
+            int sel;                                                            \
+            switch (op) {                                                       \
+                case 0: sel = c[i] > a; break;                                  \
+                case 1: sel = c[i] >= a; break;                                 \
+                case 2: sel = c[i] < a; break;                                  \
+                case 3: sel = c[i] <= a; break;                                 \
+                case 4: sel = c[i] == a; break;                                 \
+                case 5: sel = c[i] != a; break;                                 \
+                default: sel = a <= c[i] && c[i] <= b; break;                   \
+            }                                                                   \
+            if (sel) {                                                          \
+                csxo_set_bit(mask, i, bit_order);                               \
+                ones++;                                                         \
+            }                                                                   \
+        }                                                                       \
+    } while (0)
+
+uint64_t csxo_predicate_mask_typed(const void* col, uint64_t n, int type, int op, uint64_t a_bits, uint64_t b_bits, uint8_t* mask, int bit_order)
+{
+    uint64_t ones = 0;
+    if (op < 0 || op > 6) return UINT64_MAX;
+    memset(mask, 0, (n + 7) / 8);
+    switch (type) {
+        case 0: CSXO_PRED_TYPED(uint8_t); break;
+        case 1: CSXO_PRED_TYPED(uint16_t); break;
+        case 2: CSXO_PRED_TYPED(uint32_t); break;
+        case 3: CSXO_PRED_TYPED(uint64_t); break;
+        case 4: CSXO_PRED_TYPED(int8_t); break;
+        case 5: CSXO_PRED_TYPED(int16_t); break;
+        case 6: CSXO_PRED_TYPED(int32_t); break;
+        case 7: CSXO_PRED_TYPED(int64_t); break;
+        case 8: CSXO_PRED_TYPED(float); break;
+        case 9: CSXO_PRED_TYPED(double); break;
+        default: return UINT64_MAX;
+    }
+    return ones;
+}
+
 /* -------------------------------------------------------- compaction path */
 
 /* src/data_generator.cuh:171-188, including its u32 count and its walk over

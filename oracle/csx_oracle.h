@@ -76,6 +76,8 @@ void csxo_fill_values(void* buf, uint64_t n, uint32_t width, int kind, uint64_t 
 /* predicate -> mask (src/utils.cuh:102-122): bit set iff value > threshold */
 uint64_t csxo_predicate_mask_f32(const float* col, uint64_t n, float threshold, uint8_t* mask, int bit_order);
 uint64_t csxo_predicate_mask_u32(const uint32_t* col, uint64_t n, uint32_t threshold, uint8_t* mask, int bit_order);
+/* any type (0..3 u8..u64, 4..7 i8..i64, 8 f32, 9 f64) and comparison (0 >, 1 >=, 2 <, 3 <=, 4 ==, 5 !=, 6 a<=v<=b) */
+uint64_t csxo_predicate_mask_typed(const void* col, uint64_t n, int type, int op, uint64_t a_bits, uint64_t b_bits, uint8_t* mask, int bit_order);
 
 /* ---- the compaction path ---- */
 /* Exact restatement of generate_validation<T> (src/data_generator.cuh:171-188):
