@@ -275,6 +275,8 @@ def main():
         sweep["batch_4096_columns"] = bench_batch(args.steps, args.warmup)
         datas = [make_data(n, 100 + r) for r in range(ROTATE)]
 
+    sweep["predicate_f32_gt_0.5"] = bench_predicate(datas, n, K, W, out, count)
+
     if args.serve_seconds > 0:
         sweep["serving_poisson_20k"] = bench_serving(args.serve_seconds)
 
@@ -332,6 +334,47 @@ def main():
         "sweep": sweep,
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_predicate(datas, n, K, W, out, count):
+    """Column -> compacted column for `value > 0.5` over float32 in [0, 1):
+    fused (csx_compact_predicate: predicate + chunk counts, scan, writeout)
+    against two calls (csx_predicate_mask, then csx_compact), and the mask
+    step alone.  Algorithmic bytes: n*4 read + k*4 written (the mask is
+    internal); the mask step alone: n*4 read + n/8 written."""
+    mask = torch.empty((n + 7) // 8, dtype=torch.uint8, device="cuda")
+    ws_pred = csx.alloc_workspace(csx.predicate_workspace_size(n, 8))
+    ws = csx.alloc_workspace(csx.workspace_size(n, 4, 1024))
+    mcount = torch.empty(1, dtype=torch.int64, device="cuda")
+    fused_ms = time_steps(lambda i: csx.compact_predicate(datas[i % ROTATE], ">", 0.5, out=out, count=count, workspace=ws_pred, host_count=False), K, W)
+    k = int(count.item())
+
+    def two(i):
+        csx.predicate_mask(datas[i % ROTATE], ">", 0.5, mask=mask, count=mcount)
+        csx.compact(datas[i % ROTATE], mask, n=n, out=out, count=count, workspace=ws, host_count=False)
+
+    two_ms = time_steps(two, K, W)
+    mask_ms = time_steps(lambda i: csx.predicate_mask(datas[i % ROTATE], ">", 0.5, mask=mask, count=mcount), K, W)
+    # the fused result against torch on the last input
+    r = (K - 1) % ROTATE
+    _, kk = csx.compact_predicate(datas[r], ">", 0.5, out=out, count=count, workspace=ws_pred)
+    ref = torch.masked_select(datas[r], datas[r] > 0.5)
+    if kk != ref.numel() or not torch.equal(out[:kk], ref):
+        print("[bench] MISMATCH predicate compaction", file=sys.stderr)
+        sys.exit(1)
+    alg = 4 * n + 4 * k
+    res = {
+        "selected": k,
+        "fused_GB/s": round(K * alg / (fused_ms * 1e-3) / 1e9, 1),
+        "fused_ms": round(fused_ms / K, 5),
+        "two_calls_GB/s": round(K * alg / (two_ms * 1e-3) / 1e9, 1),
+        "two_calls_ms": round(two_ms / K, 5),
+        "mask_step_GB/s": round(K * (4 * n + n / 8) / (mask_ms * 1e-3) / 1e9, 1),
+        "mask_step_ms": round(mask_ms / K, 5),
+    }
+    print(f"[bench] predicate f32 > 0.5: fused {res['fused_GB/s']} GB/s ({res['fused_ms']} ms), two calls {res['two_calls_ms']} ms, "
+          f"mask step {res['mask_step_GB/s']} GB/s", file=sys.stderr, flush=True)
+    return res
 
 
 def bench_serving(seconds):
