@@ -102,6 +102,16 @@ SIGNATURES = {
     "csx_pattern_compact": (_int, [_vp, _vp, _u64, _u32, _u32, _u32, _vp, ctypes.POINTER(_u64), _vp]),
     "csx_profile_begin": (_int, []),
     "csx_profile_end": (_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
+    "csx_compact_config": (_int, [_vp, _vp, _vp, _u64, _u32, _int, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "csx_auto_workspace_size": (_int, [_u64, _u32, ctypes.POINTER(_sz)]),
+    "csx_tune_lookup": (_int, [_u32, _u64, ctypes.c_double, _vp]),
+    "csx_compact_auto": (_int, [_vp, _vp, _vp, _u64, _u32, _int, ctypes.c_double, _vp, _vp, _vp, _sz, _vp]),
+    "csx_autotune": (_int, [_u32, _u64, ctypes.c_double, _u32, _vp, ctypes.POINTER(ctypes.c_double), _vp]),
+    "csx_autotune_batch": (_int, [_vp, _u32, _u64, _u32, _int, _vp, _vp, _sz, _u32, _vp, ctypes.POINTER(ctypes.c_double), _vp]),
+    "csx_compact_batch_config": (_int, [_vp, _u32, _u64, _u32, _int, _vp, _vp, _vp, _sz, _vp]),
+    "csx_tune_save": (_int, [ctypes.c_char_p]),
+    "csx_tune_load": (_int, [ctypes.c_char_p]),
+    "csx_tune_clear": (_int, []),
     "csx_predicate_mask": (_int, [_vp, _u64, _vp, _vp, _int, _vp, _vp]),
     "csx_predicate_workspace_size": (_int, [_u64, _int, ctypes.POINTER(_sz)]),
     "csx_compact_predicate": (_int, [_vp, _u64, _vp, _vp, _vp, _int, _vp, ctypes.POINTER(_u64), _vp, _sz, _vp]),
@@ -290,6 +300,102 @@ def compact(inp, mask, n=None, chunk_len=1024, bit_order=MSB_FIRST, pipeline=PIP
         )
     )
     return out, (int(h.value) if host_count else count)
+
+
+# ---------------------------------------------------------------- tuning
+class Config(ctypes.Structure):
+    """csx_config: pipeline, chunk length, tile vectors per thread, count-kernel grid bound (batched)."""
+
+    _fields_ = [("pipeline", _int), ("chunk_len", _u32), ("vpt", _u32), ("count_blocks", _u32)]
+
+    def as_dict(self):
+        return {"pipeline": self.pipeline, "chunk_len": self.chunk_len, "vpt": self.vpt, "count_blocks": self.count_blocks}
+
+
+def auto_workspace_size(n, elem_size):
+    b = _sz(0)
+    _check(lib().csx_auto_workspace_size(n, elem_size, ctypes.byref(b)))
+    return b.value
+
+
+def compact_config(inp, mask, cfg, n=None, bit_order=MSB_FIRST, out=None, count=None, workspace=None, stream=None, host_count=True):
+    """csx_compact with an explicit Config; returns (out, k)."""
+    _require_device(inp, mask, out, count)
+    n = inp.numel() if n is None else n
+    w = inp.element_size()
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=inp.dtype, device=inp.device)
+    if count is None:
+        count = torch.empty(1, dtype=torch.int64, device=inp.device)
+    if workspace is None:
+        workspace = alloc_workspace(workspace_size(n, w, cfg.chunk_len), inp.device)
+    h = _u64(0)
+    _check(lib().csx_compact_config(_ptr(inp), _ptr(mask), _ptr(out), n, w, bit_order, ctypes.byref(cfg), _ptr(count),
+                                    ctypes.byref(h) if host_count else None, _ptr(workspace), workspace.numel(), _stream(stream)))
+    return out, (int(h.value) if host_count else count)
+
+
+def compact_auto(inp, mask, selectivity=-1.0, n=None, bit_order=MSB_FIRST, out=None, count=None, workspace=None, stream=None, host_count=True):
+    """csx_compact with the tuned configuration for (width, n, selectivity)."""
+    _require_device(inp, mask, out, count)
+    n = inp.numel() if n is None else n
+    w = inp.element_size()
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=inp.dtype, device=inp.device)
+    if count is None:
+        count = torch.empty(1, dtype=torch.int64, device=inp.device)
+    if workspace is None:
+        workspace = alloc_workspace(auto_workspace_size(n, w), inp.device)
+    h = _u64(0)
+    _check(lib().csx_compact_auto(_ptr(inp), _ptr(mask), _ptr(out), n, w, bit_order, float(selectivity), _ptr(count),
+                                  ctypes.byref(h) if host_count else None, _ptr(workspace), workspace.numel(), _stream(stream)))
+    return out, (int(h.value) if host_count else count)
+
+
+def tune_lookup(elem_size, n, selectivity=-1.0):
+    c = Config()
+    _check(lib().csx_tune_lookup(elem_size, n, float(selectivity), ctypes.byref(c)))
+    return c
+
+
+def autotune(elem_size, n, selectivity, reps=10, stream=None):
+    """Times every candidate on device-generated inputs; returns (Config, median ms)."""
+    c = Config()
+    ms = ctypes.c_double(0)
+    _check(lib().csx_autotune(elem_size, n, float(selectivity), reps, ctypes.byref(c), ctypes.byref(ms), _stream(stream)))
+    return c, ms.value
+
+
+def autotune_batch(table, elem_size, max_total, counts, workspace, reps=5, bit_order=MSB_FIRST, stream=None):
+    """Tunes the batched form on a real segment table; returns (Config, median ms)."""
+    c = Config()
+    ms = ctypes.c_double(0)
+    _check(lib().csx_autotune_batch(_ptr(table), table.shape[0], max_total, elem_size, bit_order, _ptr(counts), _ptr(workspace),
+                                    workspace.numel(), reps, ctypes.byref(c), ctypes.byref(ms), _stream(stream)))
+    return c, ms.value
+
+
+def compact_batch_config(table, elem_size, max_total, cfg, counts=None, workspace=None, bit_order=MSB_FIRST, stream=None):
+    nseg = table.shape[0]
+    if counts is None:
+        counts = torch.empty(max(nseg, 1), dtype=torch.int64, device=table.device)
+    if workspace is None:
+        workspace = alloc_workspace(batch_workspace_size(nseg, max_total, elem_size), table.device)
+    _check(lib().csx_compact_batch_config(_ptr(table), nseg, max_total, elem_size, bit_order, ctypes.byref(cfg), _ptr(counts), _ptr(workspace),
+                                          workspace.numel(), _stream(stream)))
+    return counts
+
+
+def tune_save(path):
+    _check(lib().csx_tune_save(os.fsencode(path)))
+
+
+def tune_load(path):
+    _check(lib().csx_tune_load(os.fsencode(path)))
+
+
+def tune_clear():
+    _check(lib().csx_tune_clear())
 
 
 # ---------------------------------------------------------------- predicate

@@ -8,6 +8,8 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -150,13 +152,13 @@ csx_status check_ws(const void* ws, size_t have, size_t need)
     return CSX_OK;
 }
 
-hipError_t dispatch_tile(uint32_t w, int mode, bool staged, int order, bool aligned, const csx::TileArgs& a, hipStream_t s)
+hipError_t dispatch_tile(uint32_t w, int mode, bool staged, int order, bool aligned, const csx::TileArgs& a, hipStream_t s, int vpt = 8)
 {
     switch (w) {
-        case 1: return csx::launch_tile<1>(mode, staged, order, aligned, a, s);
-        case 2: return csx::launch_tile<2>(mode, staged, order, aligned, a, s);
-        case 4: return csx::launch_tile<4>(mode, staged, order, aligned, a, s);
-        default: return csx::launch_tile<8>(mode, staged, order, aligned, a, s);
+        case 1: return csx::launch_tile<1>(mode, staged, order, aligned, a, s, vpt);
+        case 2: return csx::launch_tile<2>(mode, staged, order, aligned, a, s, vpt);
+        case 4: return csx::launch_tile<4>(mode, staged, order, aligned, a, s, vpt);
+        default: return csx::launch_tile<8>(mode, staged, order, aligned, a, s, vpt);
     }
 }
 
@@ -207,6 +209,77 @@ template <typename F> hipError_t timed(hipStream_t s, F&& fn)
     if ((e = hipEventRecord(a, s)) != hipSuccess) return e;
     if ((e = fn()) != hipSuccess) return e;
     return hipEventRecord(b, s);
+}
+
+// ---- tuning table (csx_autotune, csx_tune_*) ----
+struct TuneCell {
+    uint32_t w;
+    uint32_t log2n;
+    double sel;
+    csx_config cfg;
+    double ms;
+};
+
+struct TuneTable {
+    std::mutex mu;
+    std::vector<TuneCell> cells;
+    csx_config batch[9];  // by element width; vpt 0 = not tuned
+};
+
+TuneTable& tune_table()
+{
+    static TuneTable t;
+    return t;
+}
+
+constexpr csx_config kDefaultConfig = {CSX_PIPELINE_THREE_PHASE, 1024, 8, 0};
+
+uint32_t log2_floor(uint64_t v)
+{
+    uint32_t r = 0;
+    while (v >>= 1) r++;
+    return r;
+}
+
+csx_config lookup_config(uint32_t w, uint64_t n, double sel)
+{
+    if (sel < 0) sel = 0.5;
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    const uint32_t l = log2_floor(n ? n : 1);
+    const TuneCell* best = nullptr;
+    for (const TuneCell& c : t.cells) {
+        if (c.w != w) continue;
+        if (!best) {
+            best = &c;
+            continue;
+        }
+        const uint32_t dl = c.log2n > l ? c.log2n - l : l - c.log2n, bl = best->log2n > l ? best->log2n - l : l - best->log2n;
+        if (dl < bl || (dl == bl && std::fabs(c.sel - sel) < std::fabs(best->sel - sel))) best = &c;
+    }
+    return best ? best->cfg : kDefaultConfig;
+}
+
+void record_cell(uint32_t w, uint64_t n, double sel, const csx_config& cfg, double ms)
+{
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    const uint32_t l = log2_floor(n ? n : 1);
+    for (TuneCell& c : t.cells)
+        if (c.w == w && c.log2n == l && c.sel == sel) {
+            c.cfg = cfg;
+            c.ms = ms;
+            return;
+        }
+    t.cells.push_back(TuneCell{w, l, sel, cfg, ms});
+}
+
+csx_config batch_config(uint32_t w)
+{
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    if (w < 9 && t.batch[w].vpt) return t.batch[w];
+    return csx_config{CSX_PIPELINE_THREE_PHASE, 0, 8, 4096};
 }
 
 // csx_predicate -> one range test; returns the element size (0 if invalid)
@@ -436,13 +509,23 @@ static csx_status finish_host_count(uint64_t* h_count, const uint64_t* d_count_s
     return ok();
 }
 
-csx_status csx_compact(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, uint32_t chunk_len, int bit_order,
-                       int pipeline, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream)
+static csx_status check_vpt(uint32_t vpt, uint32_t elem_size)
+{
+    if (vpt != 2 && vpt != 4 && vpt != 8 && !(vpt == 16 && elem_size >= 2))
+        return fail(CSX_ERR_INVALID, "tile of %u vectors per thread is not built for %u-byte elements (2, 4, 8; 16 from 2 bytes)", vpt, elem_size);
+    return CSX_OK;
+}
+
+static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, uint32_t chunk_len,
+                               int bit_order, int pipeline, uint32_t vpt, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes,
+                               hipStream_t stream)
 {
     csx_status s;
     if ((s = check_width(elem_size)) || (s = check_chunk(chunk_len)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
     if (pipeline != CSX_PIPELINE_FUSED && pipeline != CSX_PIPELINE_THREE_PHASE && pipeline != CSX_PIPELINE_TWO_PASS)
         return fail(CSX_ERR_INVALID, "unknown pipeline %d", pipeline);
+    if ((s = check_vpt(vpt, elem_size))) return s;
+    if (pipeline != CSX_PIPELINE_THREE_PHASE && vpt != 8) return fail(CSX_ERR_INVALID, "only the three-phase writeout takes a tile size other than 8");
     const CompactPlan p = plan_compact(n, elem_size, chunk_len);
     if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
     if (n > 0 && (!d_in || !d_mask || !d_out)) return fail(CSX_ERR_INVALID, "null input, mask or output pointer");
@@ -505,11 +588,25 @@ csx_status csx_compact(const void* d_in, const uint8_t* d_mask, void* d_out, uin
         a.offsets = offsets;
         a.counts = counts;
         a.log2c = log2c;
-        CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeWriteout, true, bit_order, aligned, a, stream); }),
+        CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeWriteout, true, bit_order, aligned, a, stream, (int)vpt); }),
                 "csx_compact writeout");
     }
     if (h_count) return finish_host_count(h_count, count_dst, hdr, stream);
     return ok();
+}
+
+csx_status csx_compact(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, uint32_t chunk_len, int bit_order,
+                       int pipeline, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream)
+{
+    return compact_impl(d_in, d_mask, d_out, n, elem_size, chunk_len, bit_order, pipeline, 8, d_count, h_count, d_ws, ws_bytes, stream);
+}
+
+csx_status csx_compact_config(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, int bit_order,
+                              const csx_config* cfg, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream)
+{
+    if (!cfg) return fail(CSX_ERR_INVALID, "null configuration");
+    return compact_impl(d_in, d_mask, d_out, n, elem_size, cfg->chunk_len, bit_order, cfg->pipeline, cfg->vpt, d_count, h_count, d_ws, ws_bytes,
+                        stream);
 }
 
 #define CSX_COMPACT_W(W, T)                                                                                                                   \
@@ -710,10 +807,14 @@ struct BatchPlan {
     size_t prefix_off, tcounts_off, goff_off, status_off, total;
 };
 
-static BatchPlan plan_batch(uint32_t nseg, uint64_t max_total, uint32_t w)
+// the workspace is sized for the smallest tile (vpt 2), so every batched
+// configuration fits in csx_batch_workspace_size's answer
+constexpr uint32_t kBatchMinVpt = 2;
+
+static BatchPlan plan_batch(uint32_t nseg, uint64_t max_total, uint32_t w, uint32_t vpt)
 {
     BatchPlan p;
-    p.max_tiles = csx::batch_max_tiles(w, nseg, max_total);
+    p.max_tiles = csx::batch_max_tiles(w, nseg, max_total, vpt);
     p.status_words = csx::scan_tiles(p.max_tiles);
     p.prefix_off = sizeof(csx::WsHeader);
     p.tcounts_off = align_up(p.prefix_off + 8ull * (nseg + 1));
@@ -728,29 +829,44 @@ csx_status csx_batch_workspace_size(uint32_t nseg, uint64_t max_total_elements, 
     csx_status s;
     if ((s = check_width(elem_size)) || (s = check_n(max_total_elements))) return s;
     if (!bytes) return fail(CSX_ERR_INVALID, "null size pointer");
-    *bytes = plan_batch(nseg, max_total_elements, elem_size).total;
+    *bytes = plan_batch(nseg, max_total_elements, elem_size, kBatchMinVpt).total;
+    return ok();
+}
+
+static csx_status batch_impl(const csx_segment* d_segments, uint32_t nseg, uint64_t max_total_elements, uint32_t elem_size, int bit_order,
+                             const csx_config& cfg, uint64_t* d_counts, void* d_ws, size_t ws_bytes, hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_width(elem_size)) || (s = check_order(bit_order)) || (s = check_n(max_total_elements))) return s;
+    if (cfg.vpt != 2 && cfg.vpt != 4 && cfg.vpt != 8) return fail(CSX_ERR_INVALID, "batched tile of %u vectors per thread (2, 4 or 8)", cfg.vpt);
+    if (nseg == 0) return ok();
+    if (!d_segments || !d_counts) return fail(CSX_ERR_INVALID, "null segment table or counts pointer");
+    const BatchPlan p = plan_batch(nseg, max_total_elements, elem_size, cfg.vpt);
+    if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
+    if (!launch_fits(p.max_tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)p.max_tiles);
+    auto base = static_cast<uint8_t*>(d_ws);
+    const uint32_t count_blocks = cfg.count_blocks ? cfg.count_blocks : 4096;
+    CSX_HIP(timed(stream, [&] {
+                return csx::launch_batch(elem_size, bit_order, reinterpret_cast<const csx::Segment*>(d_segments), nseg, p.max_tiles, d_counts,
+                                         reinterpret_cast<uint64_t*>(base + p.prefix_off), reinterpret_cast<uint32_t*>(base + p.tcounts_off),
+                                         reinterpret_cast<uint64_t*>(base + p.goff_off), reinterpret_cast<csx::WsHeader*>(base),
+                                         reinterpret_cast<uint64_t*>(base + p.status_off), stream, cfg.vpt, count_blocks);
+            }),
+            "csx_compact_batch");
     return ok();
 }
 
 csx_status csx_compact_batch(const csx_segment* d_segments, uint32_t nseg, uint64_t max_total_elements, uint32_t elem_size, int bit_order,
                              uint64_t* d_counts, void* d_ws, size_t ws_bytes, hipStream_t stream)
 {
-    csx_status s;
-    if ((s = check_width(elem_size)) || (s = check_order(bit_order)) || (s = check_n(max_total_elements))) return s;
-    if (nseg == 0) return ok();
-    if (!d_segments || !d_counts) return fail(CSX_ERR_INVALID, "null segment table or counts pointer");
-    const BatchPlan p = plan_batch(nseg, max_total_elements, elem_size);
-    if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
-    if (!launch_fits(p.max_tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)p.max_tiles);
-    auto base = static_cast<uint8_t*>(d_ws);
-    CSX_HIP(timed(stream, [&] {
-                return csx::launch_batch(elem_size, bit_order, reinterpret_cast<const csx::Segment*>(d_segments), nseg, p.max_tiles, d_counts,
-                                         reinterpret_cast<uint64_t*>(base + p.prefix_off), reinterpret_cast<uint32_t*>(base + p.tcounts_off),
-                                         reinterpret_cast<uint64_t*>(base + p.goff_off), reinterpret_cast<csx::WsHeader*>(base),
-                                         reinterpret_cast<uint64_t*>(base + p.status_off), stream);
-            }),
-            "csx_compact_batch");
-    return ok();
+    return batch_impl(d_segments, nseg, max_total_elements, elem_size, bit_order, batch_config(elem_size), d_counts, d_ws, ws_bytes, stream);
+}
+
+csx_status csx_compact_batch_config(const csx_segment* d_segments, uint32_t nseg, uint64_t max_total_elements, uint32_t elem_size, int bit_order,
+                                    const csx_config* cfg, uint64_t* d_counts, void* d_ws, size_t ws_bytes, hipStream_t stream)
+{
+    if (!cfg) return fail(CSX_ERR_INVALID, "null configuration");
+    return batch_impl(d_segments, nseg, max_total_elements, elem_size, bit_order, *cfg, d_counts, d_ws, ws_bytes, stream);
 }
 
 // ---------------------------------------------------------------- generators
@@ -974,6 +1090,239 @@ csx_status csx_compact_predicate(const void* d_in, uint64_t n, const csx_predica
                 "csx_compact_predicate writeout");
     }
     if (h_count) return finish_host_count(h_count, count_dst, hdr, stream);
+    return ok();
+}
+
+// ---------------------------------------------------------------- tuning
+
+csx_status csx_auto_workspace_size(uint64_t n, uint32_t elem_size, size_t* bytes)
+{
+    return csx_workspace_size(n, elem_size, 256, bytes);  // the smallest chunk the tuner tries
+}
+
+csx_status csx_tune_lookup(uint32_t elem_size, uint64_t n, double selectivity, csx_config* cfg)
+{
+    csx_status s;
+    if ((s = check_width(elem_size))) return s;
+    if (!cfg) return fail(CSX_ERR_INVALID, "null configuration pointer");
+    *cfg = lookup_config(elem_size, n, selectivity);
+    return ok();
+}
+
+csx_status csx_compact_auto(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, int bit_order, double selectivity,
+                            uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_width(elem_size))) return s;
+    const csx_config c = lookup_config(elem_size, n, selectivity);
+    return compact_impl(d_in, d_mask, d_out, n, elem_size, c.chunk_len, bit_order, c.pipeline, c.vpt, d_count, h_count, d_ws, ws_bytes, stream);
+}
+
+extern "C++" {
+namespace {
+
+// Device buffers owned by one tuning call.
+struct TuneBuffers {
+    std::vector<void*> ptrs;
+    ~TuneBuffers()
+    {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    hipError_t alloc(void** p, size_t bytes)
+    {
+        hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+};
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+// median over reps of the time of fn() on stream s, after one warm-up run
+template <typename F> csx_status time_median(F&& fn, uint32_t reps, hipStream_t s, double* ms)
+{
+    EventPair ev;
+    CSX_HIP(hipEventCreate(&ev.a), "tune event");
+    CSX_HIP(hipEventCreate(&ev.b), "tune event");
+    csx_status st;
+    if ((st = fn())) return st;
+    std::vector<double> t;
+    for (uint32_t r = 0; r < reps; r++) {
+        CSX_HIP(hipEventRecord(ev.a, s), "tune record");
+        if ((st = fn())) return st;
+        CSX_HIP(hipEventRecord(ev.b, s), "tune record");
+        CSX_HIP(hipEventSynchronize(ev.b), "tune sync");
+        float f = 0.f;
+        CSX_HIP(hipEventElapsedTime(&f, ev.a, ev.b), "tune elapsed");
+        t.push_back(f);
+    }
+    std::sort(t.begin(), t.end());
+    *ms = t.empty() ? 0.0 : t[t.size() / 2];
+    return CSX_OK;
+}
+
+} // namespace
+} // extern "C++"
+
+csx_status csx_autotune(uint32_t elem_size, uint64_t n, double selectivity, uint32_t reps, csx_config* best, double* best_ms, hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_width(elem_size)) || (s = check_n(n))) return s;
+    if (!(selectivity >= 0.0 && selectivity <= 1.0)) return fail(CSX_ERR_INVALID, "selectivity must be in [0, 1]");
+    if (n == 0 || reps == 0) return fail(CSX_ERR_INVALID, "nothing to time (n = 0 or reps = 0)");
+    if (!best) return fail(CSX_ERR_INVALID, "null result pointer");
+    size_t ws_bytes = 0;
+    if ((s = csx_auto_workspace_size(n, elem_size, &ws_bytes))) return s;
+    TuneBuffers buf;
+    void *in, *mask, *out, *ws, *cnt;
+    CSX_HIP(buf.alloc(&in, n * elem_size), "tune alloc");
+    CSX_HIP(buf.alloc(&mask, (n + 7) / 8), "tune alloc");
+    CSX_HIP(buf.alloc(&out, n * elem_size), "tune alloc");
+    CSX_HIP(buf.alloc(&ws, ws_bytes), "tune alloc");
+    CSX_HIP(buf.alloc(&cnt, sizeof(uint64_t)), "tune alloc");
+    if ((s = csx_gen_values(in, n, elem_size, CSX_VALUES_INTEGER, 0x74756e65ull, stream))) return s;
+    if ((s = csx_gen_mask_bernoulli(static_cast<uint8_t*>(mask), n, selectivity, 0x6d61736bull, CSX_MSB_FIRST, stream))) return s;
+
+    std::vector<csx_config> cand;
+    for (uint32_t chunk : {256u, 1024u, 4096u})
+        for (uint32_t vpt : {2u, 4u, 8u, 16u})
+            if (vpt != 16 || elem_size >= 2) cand.push_back(csx_config{CSX_PIPELINE_THREE_PHASE, chunk, vpt, 0});
+    cand.push_back(csx_config{CSX_PIPELINE_TWO_PASS, 1024, 8, 0});
+    cand.push_back(csx_config{CSX_PIPELINE_FUSED, 1024, 8, 0});
+
+    double best_t = 0.0;
+    uint64_t first_count = 0;
+    for (size_t i = 0; i < cand.size(); i++) {
+        const csx_config c = cand[i];
+        double t = 0.0;
+        s = time_median(
+            [&] {
+                return compact_impl(in, static_cast<const uint8_t*>(mask), out, n, elem_size, c.chunk_len, CSX_MSB_FIRST, c.pipeline, c.vpt,
+                                    static_cast<uint64_t*>(cnt), nullptr, ws, ws_bytes, stream);
+            },
+            reps, stream, &t);
+        if (s) return s;
+        uint64_t k = 0;
+        if ((s = finish_host_count(&k, static_cast<uint64_t*>(cnt), static_cast<csx::WsHeader*>(ws), stream))) return s;
+        if (i == 0) first_count = k;
+        if (k != first_count) return fail(CSX_ERR_DEVICE, "configurations disagree on the selected count (%llu, %llu)", (unsigned long long)first_count,
+                                          (unsigned long long)k);
+        if (i == 0 || t < best_t) {
+            best_t = t;
+            *best = c;
+        }
+    }
+    record_cell(elem_size, n, selectivity, *best, best_t);
+    if (best_ms) *best_ms = best_t;
+    return ok();
+}
+
+csx_status csx_autotune_batch(const csx_segment* d_segments, uint32_t nseg, uint64_t max_total_elements, uint32_t elem_size, int bit_order,
+                              uint64_t* d_counts, void* d_ws, size_t ws_bytes, uint32_t reps, csx_config* best, double* best_ms, hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_width(elem_size))) return s;
+    if (!best || reps == 0) return fail(CSX_ERR_INVALID, "null result pointer or reps = 0");
+    double best_t = 0.0;
+    bool have = false;
+    for (uint32_t vpt : {2u, 4u, 8u})
+        for (uint32_t cb : {1024u, 2048u, 4096u, 8192u}) {
+            const csx_config c{CSX_PIPELINE_THREE_PHASE, 0, vpt, cb};
+            double t = 0.0;
+            s = time_median([&] { return batch_impl(d_segments, nseg, max_total_elements, elem_size, bit_order, c, d_counts, d_ws, ws_bytes, stream); },
+                            reps, stream, &t);
+            if (s) return s;
+            if (!have || t < best_t) {
+                best_t = t;
+                *best = c;
+                have = true;
+            }
+        }
+    {
+        TuneTable& t = tune_table();
+        std::lock_guard<std::mutex> g(t.mu);
+        t.batch[elem_size] = *best;
+    }
+    if (best_ms) *best_ms = best_t;
+    return ok();
+}
+
+csx_status csx_tune_save(const char* path)
+{
+    if (!path) return fail(CSX_ERR_INVALID, "null path");
+    FILE* f = fopen(path, "w");
+    if (!f) return fail(CSX_ERR_INVALID, "cannot open %s for writing", path);
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    fprintf(f, "# csx tuning table 1\n# compact <elem_size> <log2 n> <selectivity> <pipeline> <chunk_len> <vpt> <ms>\n"
+               "# batch <elem_size> <vpt> <count_blocks>\n");
+    for (const TuneCell& c : t.cells)
+        fprintf(f, "compact %u %u %.6f %d %u %u %.6f\n", c.w, c.log2n, c.sel, c.cfg.pipeline, c.cfg.chunk_len, c.cfg.vpt, c.ms);
+    for (uint32_t w : {1u, 2u, 4u, 8u})
+        if (t.batch[w].vpt) fprintf(f, "batch %u %u %u\n", w, t.batch[w].vpt, t.batch[w].count_blocks);
+    const bool bad = ferror(f) != 0;
+    if (fclose(f) != 0 || bad) return fail(CSX_ERR_INVALID, "writing %s failed", path);
+    return ok();
+}
+
+csx_status csx_tune_load(const char* path)
+{
+    if (!path) return fail(CSX_ERR_INVALID, "null path");
+    FILE* f = fopen(path, "r");
+    if (!f) return fail(CSX_ERR_INVALID, "cannot open %s", path);
+    char line[512];
+    std::vector<TuneCell> cells;
+    std::vector<std::pair<uint32_t, csx_config>> batch;
+    uint32_t lineno = 0;
+    csx_status st = CSX_OK;
+    while (fgets(line, sizeof line, f)) {
+        lineno++;
+        if (line[0] == '#' || line[0] == '\n') continue;
+        TuneCell c{};
+        uint32_t w, vpt, cb;
+        if (sscanf(line, "compact %u %u %lf %d %u %u %lf", &c.w, &c.log2n, &c.sel, &c.cfg.pipeline, &c.cfg.chunk_len, &c.cfg.vpt, &c.ms) == 7) {
+            if (check_width(c.w) || check_chunk(c.cfg.chunk_len) || check_vpt(c.cfg.vpt, c.w) || c.log2n > 40 || !(c.sel >= 0 && c.sel <= 1) ||
+                (c.cfg.pipeline != CSX_PIPELINE_THREE_PHASE && c.cfg.pipeline != CSX_PIPELINE_TWO_PASS && c.cfg.pipeline != CSX_PIPELINE_FUSED) ||
+                (c.cfg.pipeline != CSX_PIPELINE_THREE_PHASE && c.cfg.vpt != 8)) {
+                st = fail(CSX_ERR_INVALID, "%s:%u: configuration out of range", path, lineno);
+                break;
+            }
+            cells.push_back(c);
+        }
+        else if (sscanf(line, "batch %u %u %u", &w, &vpt, &cb) == 3) {
+            if (check_width(w) || (vpt != 2 && vpt != 4 && vpt != 8) || cb == 0) {
+                st = fail(CSX_ERR_INVALID, "%s:%u: batch configuration out of range", path, lineno);
+                break;
+            }
+            batch.push_back({w, csx_config{CSX_PIPELINE_THREE_PHASE, 0, vpt, cb}});
+        }
+        else {
+            st = fail(CSX_ERR_INVALID, "%s:%u: unreadable line", path, lineno);
+            break;
+        }
+    }
+    fclose(f);
+    if (st) return st;
+    for (const TuneCell& c : cells) record_cell(c.w, 1ull << c.log2n, c.sel, c.cfg, c.ms);
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    for (auto& b : batch) t.batch[b.first] = b.second;
+    return ok();
+}
+
+csx_status csx_tune_clear(void)
+{
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    t.cells.clear();
+    for (csx_config& c : t.batch) c = csx_config{0, 0, 0, 0};
     return ok();
 }
 

@@ -78,10 +78,10 @@ __device__ __forceinline__ uint32_t find_segment(const uint64_t* tile_prefix, ui
     return lo;
 }
 
-template <int W, int ORDER>
+template <int W, int ORDER, int VPT>
 __global__ __launch_bounds__(256) void k_batch_counts(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, uint32_t* tcounts)
 {
-    constexpr uint64_t TE = kTileElems<W>;
+    constexpr uint64_t TE = TileCfg<W, VPT>::TE;
     constexpr uint32_t VEC_PER_TILE = (uint32_t)(TE / 128);
     const uint32_t lane = lane_id();
     const uint64_t total = tile_prefix[nseg];
@@ -113,11 +113,11 @@ __global__ __launch_bounds__(256) void k_batch_counts(const Segment* segs, uint3
     }
 }
 
-template <int W, int ORDER>
+template <int W, int ORDER, int VPT>
 __global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, const uint64_t* goff,
                                                      uint64_t* counts)
 {
-    using C = TileCfg<W>;
+    using C = TileCfg<W, VPT>;
     __shared__ TileShared<C> sh;
     __shared__ uint32_t s_seg;
     const uint64_t g = blockIdx.x;
@@ -135,48 +135,67 @@ __global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32
     a.given_off = goff[g] - goff[first];
     a.count_out = counts + s;
     if ((((uintptr_t)a.in) | ((uintptr_t)a.mask)) & 15)
-        tile_body<W, ORDER, false, kModeGiven, true, kTileVpt<W>>(a, sh, g - first);
+        tile_body<W, ORDER, false, kModeGiven, true, VPT>(a, sh, g - first);
     else
-        tile_body<W, ORDER, true, kModeGiven, true, kTileVpt<W>>(a, sh, g - first);
+        tile_body<W, ORDER, true, kModeGiven, true, VPT>(a, sh, g - first);
 }
 
-uint64_t batch_max_tiles(uint32_t w, uint32_t nseg, uint64_t max_total)
+uint64_t batch_tile_elems(uint32_t w, uint32_t vpt) { return 256ull * vpt * (16ull / w); }
+
+uint64_t batch_max_tiles(uint32_t w, uint32_t nseg, uint64_t max_total, uint32_t vpt)
 {
-    const uint64_t te = w == 1 ? kTileElems<1> : w == 2 ? kTileElems<2> : w == 4 ? kTileElems<4> : kTileElems<8>;
-    return max_total / te + nseg;
+    return max_total / batch_tile_elems(w, vpt) + nseg;
 }
 
-template <int W, int ORDER>
+template <int W, int ORDER, int VPT>
 static hipError_t batch_impl(const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix, uint32_t* tcounts,
-                             uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s)
+                             uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks, hipStream_t s)
 {
     const uint64_t status_words = (max_tiles + kScanTile - 1) / kScanTile;
-    k_batch_plan<<<1, 256, 0, s>>>(segs, nseg, kTileElems<W>, max_tiles, tile_prefix, counts, hdr, status, status_words);
+    k_batch_plan<<<1, 256, 0, s>>>(segs, nseg, TileCfg<W, VPT>::TE, max_tiles, tile_prefix, counts, hdr, status, status_words);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || max_tiles == 0) return e;
     uint64_t g1 = (max_tiles + 3) / 4;
-    if (g1 > 4096) g1 = 4096;
-    k_batch_counts<W, ORDER><<<(unsigned)g1, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts);
+    if (g1 > count_blocks) g1 = count_blocks;
+    k_batch_counts<W, ORDER, VPT><<<(unsigned)g1, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_scan<uint64_t><<<(unsigned)status_words, kScanBlock, 0, s>>>(tcounts, goff, max_tiles, nullptr, hdr, status, tile_prefix + nseg);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_batch_write<W, ORDER><<<(unsigned)max_tiles, 256, 0, s>>>(segs, nseg, tile_prefix, goff, counts);
+    k_batch_write<W, ORDER, VPT><<<(unsigned)max_tiles, 256, 0, s>>>(segs, nseg, tile_prefix, goff, counts);
     return hipGetLastError();
 }
 
-hipError_t launch_batch(uint32_t w, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
-                        uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s)
+template <int W, int VPT>
+static hipError_t batch_order(int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
+                              uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks, hipStream_t s)
 {
-#define CSX_BATCH(W_)                                                                                   \
-    return order == kBitMsb ? batch_impl<W_, kBitMsb>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, s) \
-                            : batch_impl<W_, kBitLsb>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, s)
-    switch (w) {
-        case 1: CSX_BATCH(1);
-        case 2: CSX_BATCH(2);
-        case 4: CSX_BATCH(4);
-        default: CSX_BATCH(8);
+    return order == kBitMsb ? batch_impl<W, kBitMsb, VPT>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s)
+                            : batch_impl<W, kBitLsb, VPT>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+}
+
+template <int W>
+static hipError_t batch_vpt(uint32_t vpt, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts,
+                            uint64_t* tile_prefix, uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks,
+                            hipStream_t s)
+{
+    switch (vpt) {
+        case 2: return batch_order<W, 2>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 4: return batch_order<W, 4>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 8: return batch_order<W, 8>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        default: return hipErrorInvalidValue;
     }
-#undef CSX_BATCH
+}
+
+hipError_t launch_batch(uint32_t w, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
+                        uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s, uint32_t vpt, uint32_t count_blocks)
+{
+    if (count_blocks < 1) count_blocks = 1;
+    switch (w) {
+        case 1: return batch_vpt<1>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 2: return batch_vpt<2>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 4: return batch_vpt<4>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        default: return batch_vpt<8>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+    }
 }
 
 } // namespace csx

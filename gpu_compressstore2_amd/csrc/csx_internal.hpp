@@ -26,9 +26,13 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
 // *dst = base[i-1] + total[i-1] (0 for i = 0): the multi-stream carry
 hipError_t launch_carry(uint64_t* dst, const uint64_t* base, const uint64_t* total, uint32_t i, hipStream_t s);
 // batched compaction (csx_batch.hip)
-uint64_t batch_max_tiles(uint32_t w, uint32_t nseg, uint64_t max_total);
+// vpt in {2, 4, 8}: tile of 256 * vpt * 16/w elements; count_blocks bounds the
+// persistent grid of the per-tile count kernel
+uint64_t batch_tile_elems(uint32_t w, uint32_t vpt);
+uint64_t batch_max_tiles(uint32_t w, uint32_t nseg, uint64_t max_total, uint32_t vpt);
 hipError_t launch_batch(uint32_t w, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
-                        uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s);
+                        uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s, uint32_t vpt = 8,
+                        uint32_t count_blocks = 4096);
 // device generators (csx_gen.hip)
 hipError_t launch_gen_bernoulli(uint8_t* mask, uint64_t n, uint64_t thr, uint64_t seed, int order, hipStream_t s);
 hipError_t launch_gen_pattern(uint8_t* mask, uint64_t n, uint32_t pattern, uint32_t L, int order, hipStream_t s);
@@ -46,16 +50,18 @@ hipError_t launch_pattern(uint32_t w, const void* in, void* out, uint64_t k, uin
 hipError_t launch_validate(const void* a, const void* b, uint64_t len, uint32_t w, uint64_t* failures, hipStream_t s);
 
 // tile engine, instantiated once per element width (csx_tile.hip)
-template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool aligned, const TileArgs& a, hipStream_t s);
+// vpt: 16-byte vectors per thread of the three-phase writeout tile (2, 4, 8, or 16
+// for W >= 2); the other modes use kTileVpt
+template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool aligned, const TileArgs& a, hipStream_t s, int vpt = kTileVpt<W>);
 template <int W>
 hipError_t launch_tile_counts(int order, bool aligned, const uint8_t* mask, uint64_t n, uint32_t* tcounts, uint64_t* gsums, uint32_t log2g,
                               WsHeader* hdr, hipStream_t s);
 template <int W> hipError_t launch_thread_per_chunk(int order, const TileArgs& a, hipStream_t s);
 
-extern template hipError_t launch_tile<1>(int, bool, int, bool, const TileArgs&, hipStream_t);
-extern template hipError_t launch_tile<2>(int, bool, int, bool, const TileArgs&, hipStream_t);
-extern template hipError_t launch_tile<4>(int, bool, int, bool, const TileArgs&, hipStream_t);
-extern template hipError_t launch_tile<8>(int, bool, int, bool, const TileArgs&, hipStream_t);
+extern template hipError_t launch_tile<1>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
+extern template hipError_t launch_tile<2>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
+extern template hipError_t launch_tile<4>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
+extern template hipError_t launch_tile<8>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
 extern template hipError_t launch_tile_counts<1>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, WsHeader*, hipStream_t);
 extern template hipError_t launch_thread_per_chunk<1>(int, const TileArgs&, hipStream_t);
 extern template hipError_t launch_tile_counts<2>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, WsHeader*, hipStream_t);

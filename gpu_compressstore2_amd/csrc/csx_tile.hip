@@ -9,8 +9,14 @@
 
 namespace csx {
 
+template <int W, int ORDER, bool ALIGNED, int VPT> static void launch_writeout_vpt(const TileArgs& a, hipStream_t s)
+{
+    const uint64_t tiles = (a.n + TileCfg<W, VPT>::TE - 1) / TileCfg<W, VPT>::TE;
+    k_tile<W, ORDER, ALIGNED, kModeWriteout, true, VPT><<<(unsigned)tiles, 256, 0, s>>>(a);
+}
+
 template <int W, int ORDER, bool ALIGNED>
-static void launch_tile_impl(int mode, bool staged, const TileArgs& a, unsigned grid, hipStream_t s)
+static void launch_tile_impl(int mode, bool staged, int vpt, const TileArgs& a, unsigned grid, hipStream_t s)
 {
     if (mode == kModeTwoPass) {
         k_tile<W, ORDER, ALIGNED, kModeTwoPass, true><<<grid, 256, 0, s>>>(a);
@@ -19,29 +25,39 @@ static void launch_tile_impl(int mode, bool staged, const TileArgs& a, unsigned 
         k_tile<W, ORDER, ALIGNED, kModeFused, true><<<grid, 256, 0, s>>>(a);
     }
     else if (staged) {
-        k_tile<W, ORDER, ALIGNED, kModeWriteout, true><<<grid, 256, 0, s>>>(a);
+        // three-phase writeout: a tile's offset comes from the chunk offsets,
+        // so any tile size works with the same workspace
+        switch (vpt) {
+            case 2: launch_writeout_vpt<W, ORDER, ALIGNED, 2>(a, s); break;
+            case 4: launch_writeout_vpt<W, ORDER, ALIGNED, 4>(a, s); break;
+            case 16:
+                if constexpr (W >= 2) launch_writeout_vpt<W, ORDER, ALIGNED, 16>(a, s);
+                break;
+            default: k_tile<W, ORDER, ALIGNED, kModeWriteout, true><<<grid, 256, 0, s>>>(a); break;
+        }
     }
     else {
         k_tile<W, ORDER, ALIGNED, kModeWriteout, false><<<grid, 256, 0, s>>>(a);
     }
 }
 
-template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool aligned, const TileArgs& a, hipStream_t s)
+template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool aligned, const TileArgs& a, hipStream_t s, int vpt)
 {
+    if (vpt != 2 && vpt != 4 && vpt != 8 && !(vpt == 16 && W >= 2)) return hipErrorInvalidValue;
     const uint64_t tiles = (a.n + kTileElems<W> - 1) / kTileElems<W>;
     if (tiles == 0) return hipSuccess;
     const unsigned grid = (unsigned)tiles;
     if (order == kBitMsb) {
         if (aligned)
-            launch_tile_impl<W, kBitMsb, true>(mode, staged, a, grid, s);
+            launch_tile_impl<W, kBitMsb, true>(mode, staged, vpt, a, grid, s);
         else
-            launch_tile_impl<W, kBitMsb, false>(mode, staged, a, grid, s);
+            launch_tile_impl<W, kBitMsb, false>(mode, staged, vpt, a, grid, s);
     }
     else {
         if (aligned)
-            launch_tile_impl<W, kBitLsb, true>(mode, staged, a, grid, s);
+            launch_tile_impl<W, kBitLsb, true>(mode, staged, vpt, a, grid, s);
         else
-            launch_tile_impl<W, kBitLsb, false>(mode, staged, a, grid, s);
+            launch_tile_impl<W, kBitLsb, false>(mode, staged, vpt, a, grid, s);
     }
     return hipGetLastError();
 }
@@ -81,7 +97,7 @@ template <int W> hipError_t launch_thread_per_chunk(int order, const TileArgs& a
     return hipGetLastError();
 }
 
-template hipError_t launch_tile<CSX_TILE_W>(int, bool, int, bool, const TileArgs&, hipStream_t);
+template hipError_t launch_tile<CSX_TILE_W>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
 template hipError_t launch_tile_counts<CSX_TILE_W>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, WsHeader*, hipStream_t);
 template hipError_t launch_thread_per_chunk<CSX_TILE_W>(int, const TileArgs&, hipStream_t);
 

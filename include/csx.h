@@ -263,6 +263,54 @@ csx_status csx_server_poll(csx_server* server, csx_completion* done, uint32_t ca
 uint32_t csx_server_queued(const csx_server* server);
 uint32_t csx_server_inflight(const csx_server* server);
 
+/* ---------------------------------------------------------------- tuning
+ * A compaction configuration.  pipeline: CSX_PIPELINE_*; chunk_len: the
+ * three-phase chunk length; vpt: tile size in 16-byte vectors per thread of
+ * the three-phase writeout (2, 4, 8, or 16 when elem_size >= 2; TWO_PASS and
+ * FUSED always use 8) or of the batched form (2, 4 or 8); count_blocks: grid
+ * bound of the batched form's per-tile count kernel (0: 4096).
+ * The tuning table maps (elem_size, log2 n, selectivity) to the fastest
+ * measured configuration; csx_tune_lookup answers with the measured cell of
+ * that width nearest in log2 n, then in selectivity, or with the default
+ * (THREE_PHASE, 1024, 8) when none was measured.  No counterpart in the
+ * reference, whose block size and chunk length are benchmark arguments
+ * (src/main.cu:38-46, src/kernels/kernel_3pass.cuh:491-533). */
+typedef struct {
+    int pipeline;
+    uint32_t chunk_len;
+    uint32_t vpt;
+    uint32_t count_blocks;
+} csx_config;
+/* csx_compact with an explicit configuration; workspace csx_workspace_size(n, elem_size, cfg->chunk_len) */
+csx_status csx_compact_config(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, int bit_order,
+                              const csx_config* cfg, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream);
+/* workspace that fits every configuration the tuner may choose */
+csx_status csx_auto_workspace_size(uint64_t n, uint32_t elem_size, size_t* bytes);
+/* selectivity in [0, 1], or < 0 when unknown (then 0.5) */
+csx_status csx_tune_lookup(uint32_t elem_size, uint64_t n, double selectivity, csx_config* cfg);
+csx_status csx_compact_auto(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, int bit_order, double selectivity,
+                            uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream);
+/* Autotuner for one cell: generates n elements and a Bernoulli(selectivity)
+ * mask on the device, times every candidate (THREE_PHASE x chunk {256, 1024,
+ * 4096} x vpt {2, 4, 8, 16}, TWO_PASS, FUSED) with HIP events -- the median
+ * of `reps` runs after one warm-up -- checks that all give the same count,
+ * records the fastest in the table and returns it.  Allocates and frees its
+ * own device buffers and synchronises the stream: an offline step. */
+csx_status csx_autotune(uint32_t elem_size, uint64_t n, double selectivity, uint32_t reps, csx_config* best, double* best_ms, hipStream_t stream);
+/* The batched form tuned on the caller's own table (vpt {2, 4, 8} x
+ * count_blocks {1024, 2048, 4096, 8192}); the result is recorded for
+ * elem_size and used by csx_compact_batch from then on.  Runs the batch
+ * reps+1 times per candidate (outputs and counts are rewritten each time). */
+csx_status csx_autotune_batch(const csx_segment* d_segments, uint32_t nseg, uint64_t max_total_elements, uint32_t elem_size, int bit_order,
+                              uint64_t* d_counts, void* d_ws, size_t ws_bytes, uint32_t reps, csx_config* best, double* best_ms,
+                              hipStream_t stream);
+csx_status csx_compact_batch_config(const csx_segment* d_segments, uint32_t nseg, uint64_t max_total_elements, uint32_t elem_size, int bit_order,
+                                    const csx_config* cfg, uint64_t* d_counts, void* d_ws, size_t ws_bytes, hipStream_t stream);
+/* the table as text (one measured cell a line); load adds to the table */
+csx_status csx_tune_save(const char* path);
+csx_status csx_tune_load(const char* path);
+csx_status csx_tune_clear(void);
+
 /* ---------------------------------------------------------------- predicate
  * The step before the path: a column and a comparison give the packed mask.
  * csx_predicate_mask replaces gen_predicate (src/utils.cuh:102-122) with the

@@ -1,0 +1,83 @@
+"""CPU checks of the tuning table: text round trip, nearest-cell lookup,
+rejected lines, and configuration checks that fail before any device work."""
+import ctypes
+
+import pytest
+
+import gpu_compressstore2_amd as csx
+
+
+@pytest.fixture(autouse=True)
+def clean_table():
+    csx.tune_clear()
+    yield
+    csx.tune_clear()
+
+
+def test_default_without_measurements():
+    c = csx.tune_lookup(4, 1 << 28, 0.5)
+    assert (c.pipeline, c.chunk_len, c.vpt) == (csx.PIPELINE_THREE_PHASE, 1024, 8)
+
+
+def test_save_load_and_nearest_cell(tmp_path):
+    p = tmp_path / "t.txt"
+    p.write_text(
+        "# a table\n"
+        "compact 4 20 0.500000 1 256 2 0.050\n"
+        "compact 4 28 0.500000 1 1024 8 0.280\n"
+        "compact 4 28 0.990000 2 1024 8 0.380\n"
+        "compact 1 28 0.500000 1 4096 4 0.300\n"
+        "batch 4 4 2048\n"
+    )
+    csx.tune_load(str(p))
+    c = csx.tune_lookup(4, 1 << 20, 0.5)
+    assert (c.pipeline, c.chunk_len, c.vpt) == (1, 256, 2)
+    c = csx.tune_lookup(4, 3 << 27, 0.97)  # log2 n = 28, nearest selectivity 0.99
+    assert (c.pipeline, c.chunk_len, c.vpt) == (2, 1024, 8)
+    c = csx.tune_lookup(4, 1 << 27, -1)  # unknown selectivity: 0.5; nearest size 28
+    assert (c.pipeline, c.chunk_len, c.vpt) == (1, 1024, 8)
+    c = csx.tune_lookup(1, 1 << 10, 0.1)  # the only cell of width 1
+    assert (c.chunk_len, c.vpt) == (4096, 4)
+    c = csx.tune_lookup(2, 1 << 28, 0.5)  # nothing for width 2
+    assert (c.chunk_len, c.vpt) == (1024, 8)
+    q = tmp_path / "u.txt"
+    csx.tune_save(str(q))
+    text = q.read_text()
+    assert text.count("\ncompact ") == 4 and "batch 4 4 2048" in text
+    csx.tune_clear()
+    csx.tune_load(str(q))
+    assert csx.tune_lookup(4, 1 << 20, 0.5).vpt == 2
+
+
+@pytest.mark.parametrize(
+    "line",
+    [
+        "compact 3 20 0.5 1 1024 8 0.1",  # width
+        "compact 4 20 0.5 1 1000 8 0.1",  # chunk not a power of two
+        "compact 1 20 0.5 1 1024 16 0.1",  # vpt 16 needs 2-byte elements
+        "compact 4 20 0.5 2 1024 4 0.1",  # two-pass uses vpt 8
+        "compact 4 20 1.5 1 1024 8 0.1",  # selectivity
+        "batch 4 16 4096",  # batched vpt
+        "hello",
+    ],
+)
+def test_bad_lines_rejected(tmp_path, line):
+    p = tmp_path / "bad.txt"
+    p.write_text(line + "\n")
+    with pytest.raises(csx.CsxError):
+        csx.tune_load(str(p))
+    assert csx.tune_lookup(4, 1 << 20, 0.5).vpt == 8  # nothing half-loaded
+
+
+def test_config_checks_before_device_work():
+    L = csx.lib()
+    for pipeline, chunk, vpt, w in [(1, 1024, 3, 4), (1, 1024, 16, 1), (2, 1024, 4, 4), (7, 1024, 8, 4), (1, 100, 8, 4)]:
+        c = csx.Config(pipeline, chunk, vpt, 0)
+        assert L.csx_compact_config(None, None, None, 10, w, 0, ctypes.byref(c), None, None, None, 0, None) == 1
+    assert L.csx_compact_config(None, None, None, 10, 4, 0, None, None, None, None, 0, None) == 1
+    c = csx.Config(1, 0, 16, 0)
+    assert L.csx_compact_batch_config(None, 1, 10, 4, 0, ctypes.byref(c), None, None, 0, None) == 1
+    ms = ctypes.c_double()
+    assert L.csx_autotune(4, 0, 0.5, 3, ctypes.byref(c), ctypes.byref(ms), None) == 1
+    assert L.csx_autotune(4, 100, 1.5, 3, ctypes.byref(c), ctypes.byref(ms), None) == 1
+    assert csx.auto_workspace_size(1 << 20, 4) == csx.workspace_size(1 << 20, 4, 256)
