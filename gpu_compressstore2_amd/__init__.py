@@ -22,6 +22,7 @@ import torch
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libcsx.so")
+TUNED_TABLE = os.path.join(PKG, "tuned_mi355x.txt")
 
 MSB_FIRST = 0
 LSB_FIRST = 1
@@ -146,6 +147,9 @@ def lib():
                 f.restype = res
                 f.argtypes = args
         _LIB = L
+        # the measured tuning table of this GPU model, when shipped
+        if os.path.exists(TUNED_TABLE) and L.csx_tune_load(os.fsencode(TUNED_TABLE)) != 0:
+            raise CsxError(1, f"{TUNED_TABLE}: {L.csx_last_error().decode()}")
     return _LIB
 
 
