@@ -117,9 +117,19 @@ def bench_batch(steps, warmup, seed=2024, nseg=4096):
     step_bytes = float(4 * total + total / 8 + 4 * ks.sum())
     res = {"columns": nseg, "total_elements": total, "algorithmic_bytes": int(step_bytes)}
 
+    # the default tile (8 vectors per thread), then the autotuner on this table
+    cfg8 = csx.Config(csx.PIPELINE_THREE_PHASE, 0, 8, 4096)
+    ms_d = time_steps(lambda i: csx.compact_batch_config(table, 4, total, cfg8, counts=counts, workspace=ws), steps, warmup)
+    res["default_config_GB/s"] = round(steps * step_bytes / (ms_d * 1e-3) / 1e9, 1)
+    tuned, _ = csx.autotune_batch(table, 4, total, counts, ws, reps=5)
+    res["tuned_config"] = tuned.as_dict()
     ms_b = time_steps(lambda i: csx.compact_batch(table, 4, total, counts=counts, workspace=ws), steps, warmup)
     res["batched_GB/s"] = round(steps * step_bytes / (ms_b * 1e-3) / 1e9, 1)
     res["batched_ms"] = round(ms_b / steps, 4)
+    torch.cuda.synchronize()
+    if not np.array_equal(counts.cpu().numpy(), ks):
+        print("[bench] MISMATCH batched counts with the tuned configuration", file=sys.stderr)
+        sys.exit(1)
 
     # one csx_compact per column, through the C ABI with prebuilt arguments
     L = csx.lib()
@@ -245,6 +255,19 @@ def main():
             del expect
             if name == "three_phase" and p == 0.5:
                 main_result = (gbs, ms / K, kern_avg, step_bytes)
+        # the tuned configuration for this width, size and selectivity
+        ws_auto = csx.alloc_workspace(csx.auto_workspace_size(n, w))
+        ms = time_steps(lambda i: csx.compact_auto(datas[i % ROTATE], masks[i % ROTATE], p, n=n, out=out, count=count, workspace=ws_auto,
+                                                   host_count=False), K, W)
+        gbs = K * step_bytes / (ms * 1e-3) / 1e9
+        sweep[f"auto@{p}"] = {"GB/s": round(gbs, 1), "pct_peak": round(100 * gbs / HBM_PEAK_GBS, 2), "ms_per_step": round(ms / K, 5),
+                              "config": csx.tune_lookup(w, n, p).as_dict()}
+        print(f"[bench] auto p={p}: {gbs:.1f} GB/s ({ms / K:.4f} ms/step) {sweep[f'auto@{p}']['config']}", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        if int(count.item()) != ks[(K - 1) % ROTATE]:
+            print(f"[bench] auto count mismatch at p={p}", file=sys.stderr)
+            sys.exit(1)
+        del ws_auto
         # vendor baseline on the same inputs: rocPRIM flagged select
         temp = torch.empty(max(baseline.temp_size(w, n), 16), dtype=torch.uint8, device="cuda")
         ms = time_steps(lambda i: baseline.rocprim_select(datas[i % ROTATE], masks[i % ROTATE], n=n, out=out, count=count, temp=temp), K, W)
