@@ -20,14 +20,20 @@ namespace {
 
 thread_local std::string g_last_error;
 
-csx_status fail(csx_status s, const char* fmt, ...)
+csx_status vfail(csx_status s, const char* fmt, va_list ap)
 {
     char buf[512];
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    g_last_error = buf;
+    return s;
+}
+
+csx_status fail(csx_status s, const char* fmt, ...)
+{
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
+    vfail(s, fmt, ap);
     va_end(ap);
-    g_last_error = buf;
     return s;
 }
 
@@ -320,6 +326,22 @@ size_t pred_mask_off(uint64_t n, uint32_t w) { return align_up(plan_compact(n, w
 size_t pred_ws_total(uint64_t n, uint32_t w) { return pred_mask_off(n, w) + align_up((n + 7) / 8); }
 
 } // namespace
+
+namespace csx {
+csx_status set_error(csx_status s, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vfail(s, fmt, ap);
+    va_end(ap);
+    return s;
+}
+csx_status clear_error()
+{
+    g_last_error.clear();
+    return CSX_OK;
+}
+} // namespace csx
 
 extern "C" {
 

@@ -3,9 +3,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include "csx.h"
+
 #include "csx_types.hpp"
 
 namespace csx {
+
+// the calling thread's csx_last_error() message (csx_api.cpp)
+csx_status set_error(csx_status s, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+csx_status clear_error();
 
 // With `hdr` set, the popcount kernel also initialises the look-back state of
 // `init_tiles` scan tiles (saves the init launch before launch_scan).

@@ -15,6 +15,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "csx_internal.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -49,31 +51,33 @@ static constexpr uint64_t kArenaAlign = 256;
 
 extern "C" csx_status csx_arena_create(uint64_t bytes, csx_arena** out)
 {
-    if (!out || bytes == 0) return CSX_ERR_INVALID;
+    if (!out || bytes == 0) return csx::set_error(CSX_ERR_INVALID, "arena: null result pointer or zero bytes");
     auto* a = new (std::nothrow) csx_arena();
-    if (!a) return CSX_ERR_INVALID;
+    if (!a) return csx::set_error(CSX_ERR_INVALID, "arena: out of host memory");
     a->bytes = (bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
-    if (hipMalloc(&a->base, a->bytes) != hipSuccess) {
+    const hipError_t e = hipMalloc(&a->base, a->bytes);
+    if (e != hipSuccess) {
         delete a;
-        return CSX_ERR_HIP;
+        return csx::set_error(CSX_ERR_HIP, "arena: hipMalloc of %llu bytes: %s", (unsigned long long)bytes, hipGetErrorString(e));
     }
     a->free_list[0] = a->bytes;
     *out = a;
-    return CSX_OK;
+    return csx::clear_error();
 }
 
 extern "C" csx_status csx_arena_destroy(csx_arena* a)
 {
-    if (!a) return CSX_ERR_INVALID;
+    if (!a) return csx::set_error(CSX_ERR_INVALID, "arena: null handle");
     hipError_t e = hipFree(a->base);
     delete a;
-    return e == hipSuccess ? CSX_OK : CSX_ERR_HIP;
+    if (e != hipSuccess) return csx::set_error(CSX_ERR_HIP, "arena: hipFree: %s", hipGetErrorString(e));
+    return csx::clear_error();
 }
 
 // first fit; CSX_ERR_WORKSPACE when no free range is large enough
 extern "C" csx_status csx_arena_alloc(csx_arena* a, uint64_t bytes, void** ptr)
 {
-    if (!a || !ptr) return CSX_ERR_INVALID;
+    if (!a || !ptr) return csx::set_error(CSX_ERR_INVALID, "arena: null handle or result pointer");
     const uint64_t need = std::max<uint64_t>(kArenaAlign, (bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign);
     std::lock_guard<std::mutex> lk(a->mu);
     for (auto it = a->free_list.begin(); it != a->free_list.end(); ++it) {
@@ -87,16 +91,19 @@ extern "C" csx_status csx_arena_alloc(csx_arena* a, uint64_t bytes, void** ptr)
             return CSX_OK;
         }
     }
-    return CSX_ERR_WORKSPACE;
+    return csx::set_error(CSX_ERR_WORKSPACE, "arena: no free range of %llu bytes (%llu of %llu in use)", (unsigned long long)need,
+                          (unsigned long long)a->used, (unsigned long long)a->bytes);
 }
 
 extern "C" csx_status csx_arena_free(csx_arena* a, void* ptr)
 {
-    if (!a || !ptr) return CSX_ERR_INVALID;
+    if (!a || !ptr) return csx::set_error(CSX_ERR_INVALID, "arena: null handle or pointer");
     std::lock_guard<std::mutex> lk(a->mu);
-    const uint64_t off = (uint64_t)(static_cast<uint8_t*>(ptr) - a->base);
+    const uint8_t* p = static_cast<uint8_t*>(ptr);
+    if (p < a->base || p >= a->base + a->bytes) return csx::set_error(CSX_ERR_INVALID, "arena: pointer outside the arena");
+    const uint64_t off = (uint64_t)(p - a->base);
     auto it = a->live.find(off);
-    if (it == a->live.end()) return CSX_ERR_INVALID;
+    if (it == a->live.end()) return csx::set_error(CSX_ERR_INVALID, "arena: pointer is not a live allocation");
     uint64_t start = off, size = it->second;
     a->live.erase(it);
     a->used -= size;
@@ -133,11 +140,13 @@ struct Batch {
     int slot = -1;
     uint32_t width = 0;
     std::vector<Pending> reqs;
-    hipEvent_t done = nullptr;
 };
 
+// A stream with its own table, counts and workspace; at most one batch in
+// flight, so one completion event per slot.
 struct Slot {
     hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
     csx_segment* h_table = nullptr;  // pinned
     csx_segment* d_table = nullptr;
     uint64_t* d_counts = nullptr;
@@ -160,9 +169,11 @@ struct csx_server {
 
 extern "C" csx_status csx_server_create(const csx_server_config* cfg, csx_server** out)
 {
-    if (!cfg || !out || cfg->streams < 1 || cfg->streams > 16 || cfg->max_batch < 1 || cfg->max_batch_elements == 0) return CSX_ERR_INVALID;
+    if (!cfg || !out) return csx::set_error(CSX_ERR_INVALID, "server: null configuration or result pointer");
+    if (cfg->streams < 1 || cfg->streams > 16 || cfg->max_batch < 1 || cfg->max_batch_elements == 0)
+        return csx::set_error(CSX_ERR_INVALID, "server: streams must be 1..16, max_batch and max_batch_elements positive");
     auto* s = new (std::nothrow) csx_server();
-    if (!s) return CSX_ERR_INVALID;
+    if (!s) return csx::set_error(CSX_ERR_INVALID, "server: out of host memory");
     s->cfg = *cfg;
     s->slots.resize(cfg->streams);
     size_t ws = 0;
@@ -176,30 +187,28 @@ extern "C" csx_status csx_server_create(const csx_server_config* cfg, csx_server
         ws = std::max(ws, b);
     }
     for (auto& sl : s->slots) {
-        bool ok = hipStreamCreate(&sl.stream) == hipSuccess && hipHostMalloc(&sl.h_table, sizeof(csx_segment) * cfg->max_batch) == hipSuccess &&
+        bool ok = hipStreamCreate(&sl.stream) == hipSuccess && hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
+                  hipHostMalloc(&sl.h_table, sizeof(csx_segment) * cfg->max_batch) == hipSuccess &&
                   hipMalloc(&sl.d_table, sizeof(csx_segment) * cfg->max_batch) == hipSuccess &&
                   hipMalloc(&sl.d_counts, sizeof(uint64_t) * cfg->max_batch) == hipSuccess &&
                   hipHostMalloc(&sl.h_counts, sizeof(uint64_t) * cfg->max_batch) == hipSuccess && hipMalloc(&sl.d_ws, ws) == hipSuccess;
         sl.ws_bytes = ws;
         if (!ok) {
             csx_server_destroy(s);
-            return CSX_ERR_HIP;
+            return csx::set_error(CSX_ERR_HIP, "server: creating stream %zu's resources failed", (size_t)(&sl - s->slots.data()));
         }
     }
     *out = s;
-    return CSX_OK;
+    return csx::clear_error();
 }
 
+// Waits for in-flight batches; their completions are dropped.
 extern "C" csx_status csx_server_destroy(csx_server* s)
 {
-    if (!s) return CSX_ERR_INVALID;
-    for (auto& b : s->inflight)
-        if (b.done) {
-            (void)hipEventSynchronize(b.done);
-            (void)hipEventDestroy(b.done);
-        }
+    if (!s) return csx::set_error(CSX_ERR_INVALID, "server: null handle");
     for (auto& sl : s->slots) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.h_table) (void)hipHostFree(sl.h_table);
         if (sl.d_table) (void)hipFree(sl.d_table);
         if (sl.d_counts) (void)hipFree(sl.d_counts);
@@ -208,17 +217,20 @@ extern "C" csx_status csx_server_destroy(csx_server* s)
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     delete s;
-    return CSX_OK;
+    return csx::clear_error();
 }
 
 extern "C" csx_status csx_server_submit(csx_server* s, const csx_request* r)
 {
-    if (!s || !r) return CSX_ERR_INVALID;
-    if (r->elem_size != 1 && r->elem_size != 2 && r->elem_size != 4 && r->elem_size != 8) return CSX_ERR_INVALID;
-    if (r->n > s->cfg.max_batch_elements) return CSX_ERR_INVALID;
-    if (r->n > 0 && (!r->in || !r->mask || !r->out)) return CSX_ERR_INVALID;
+    if (!s || !r) return csx::set_error(CSX_ERR_INVALID, "server: null handle or request");
+    if (r->elem_size != 1 && r->elem_size != 2 && r->elem_size != 4 && r->elem_size != 8)
+        return csx::set_error(CSX_ERR_INVALID, "server: element size %u must be 1, 2, 4 or 8", r->elem_size);
+    if (r->n > s->cfg.max_batch_elements)
+        return csx::set_error(CSX_ERR_INVALID, "server: request of %llu elements exceeds max_batch_elements %llu", (unsigned long long)r->n,
+                              (unsigned long long)s->cfg.max_batch_elements);
+    if (r->n > 0 && (!r->in || !r->mask || !r->out)) return csx::set_error(CSX_ERR_INVALID, "server: null input, mask or output");
     s->queue.push_back(Pending{*r, now_ms()});
-    return CSX_OK;
+    return csx::clear_error();
 }
 
 extern "C" uint32_t csx_server_queued(const csx_server* s) { return s ? (uint32_t)s->queue.size() : 0; }
@@ -226,10 +238,31 @@ extern "C" uint32_t csx_server_inflight(const csx_server* s) { return s ? (uint3
 
 // Launch batches from the queue while stream slots are free: each batch takes
 // the oldest queued request's width and every queued request of that width,
-// up to max_batch requests and max_batch_elements elements.
+// up to max_batch requests and max_batch_elements elements.  A failed launch
+// puts the batch's requests back at the front of the queue.
+static csx_status launch_batch(csx_server* s, Slot& sl, const Batch& b)
+{
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < b.reqs.size() && e == hipSuccess; i++) {
+        const csx_request& r = b.reqs[i].req;
+        sl.h_table[i] = csx_segment{r.in, r.mask, r.out, r.n};
+        if (r.ready) e = hipStreamWaitEvent(sl.stream, r.ready, 0);
+    }
+    const uint32_t nseg = (uint32_t)b.reqs.size();
+    if (e == hipSuccess) e = hipMemcpyAsync(sl.d_table, sl.h_table, sizeof(csx_segment) * nseg, hipMemcpyHostToDevice, sl.stream);
+    if (e != hipSuccess) return csx::set_error(CSX_ERR_HIP, "server: batch table upload: %s", hipGetErrorString(e));
+    const csx_status st = csx_compact_batch(sl.d_table, nseg, s->cfg.max_batch_elements, b.width, CSX_MSB_FIRST, sl.d_counts, sl.d_ws, sl.ws_bytes,
+                                            sl.stream);
+    if (st != CSX_OK) return st;
+    e = hipMemcpyAsync(sl.h_counts, sl.d_counts, sizeof(uint64_t) * nseg, hipMemcpyDeviceToHost, sl.stream);
+    if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
+    if (e != hipSuccess) return csx::set_error(CSX_ERR_HIP, "server: batch completion: %s", hipGetErrorString(e));
+    return CSX_OK;
+}
+
 extern "C" csx_status csx_server_step(csx_server* s, uint32_t* launched)
 {
-    if (!s) return CSX_ERR_INVALID;
+    if (!s) return csx::set_error(CSX_ERR_INVALID, "server: null handle");
     uint32_t nl = 0;
     while (!s->queue.empty()) {
         int slot = -1;
@@ -256,19 +289,12 @@ extern "C" csx_status csx_server_step(csx_server* s, uint32_t* launched)
                 ++it;
             }
         }
-        for (size_t i = 0; i < b.reqs.size(); i++) {
-            const csx_request& r = b.reqs[i].req;
-            sl.h_table[i] = csx_segment{r.in, r.mask, r.out, r.n};
-            if (r.ready && hipStreamWaitEvent(sl.stream, r.ready, 0) != hipSuccess) return CSX_ERR_HIP;
+        const csx_status st = launch_batch(s, sl, b);
+        if (st != CSX_OK) {
+            s->queue.insert(s->queue.begin(), b.reqs.begin(), b.reqs.end());
+            if (launched) *launched = nl;
+            return st;
         }
-        const uint32_t nseg = (uint32_t)b.reqs.size();
-        if (hipMemcpyAsync(sl.d_table, sl.h_table, sizeof(csx_segment) * nseg, hipMemcpyHostToDevice, sl.stream) != hipSuccess) return CSX_ERR_HIP;
-        csx_status st = csx_compact_batch(sl.d_table, nseg, s->cfg.max_batch_elements, b.width, CSX_MSB_FIRST, sl.d_counts, sl.d_ws, sl.ws_bytes,
-                                          sl.stream);
-        if (st != CSX_OK) return st;
-        if (hipMemcpyAsync(sl.h_counts, sl.d_counts, sizeof(uint64_t) * nseg, hipMemcpyDeviceToHost, sl.stream) != hipSuccess) return CSX_ERR_HIP;
-        if (hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess) return CSX_ERR_HIP;
-        if (hipEventRecord(b.done, sl.stream) != hipSuccess) return CSX_ERR_HIP;
         sl.busy = true;
         s->next_slot = (uint32_t)(slot + 1) % s->slots.size();
         s->inflight.push_back(std::move(b));
@@ -276,34 +302,38 @@ extern "C" csx_status csx_server_step(csx_server* s, uint32_t* launched)
         nl++;
     }
     if (launched) *launched = nl;
-    return CSX_OK;
+    return csx::clear_error();
 }
 
-// Retire finished batches (any order) without blocking.
+// Retire finished batches (any order) without blocking.  `cap` must hold a
+// whole batch (at least max_batch completions).
 extern "C" csx_status csx_server_poll(csx_server* s, csx_completion* out, uint32_t cap, uint32_t* n)
 {
-    if (!s || !n || (cap && !out)) return CSX_ERR_INVALID;
+    if (!s || !n || !out) return csx::set_error(CSX_ERR_INVALID, "server: null handle, completion buffer or count pointer");
+    if (cap < s->cfg.max_batch) return csx::set_error(CSX_ERR_INVALID, "server: capacity %u is below max_batch %u", cap, s->cfg.max_batch);
     uint32_t k = 0;
     for (auto it = s->inflight.begin(); it != s->inflight.end();) {
         if (k + it->reqs.size() > cap) break;
-        const hipError_t q = hipEventQuery(it->done);
+        Slot& sl = s->slots[it->slot];
+        const hipError_t q = hipEventQuery(sl.done);
         if (q == hipErrorNotReady) {
             ++it;
             continue;
         }
-        if (q != hipSuccess) return CSX_ERR_HIP;
+        if (q != hipSuccess) {
+            *n = k;
+            return csx::set_error(CSX_ERR_HIP, "server: batch on stream %d: %s", it->slot, hipGetErrorString(q));
+        }
         const double t = now_ms();
-        Slot& sl = s->slots[it->slot];
         for (size_t i = 0; i < it->reqs.size(); i++) {
             out[k].id = it->reqs[i].req.id;
             out[k].count = sl.h_counts[i];
             out[k].latency_ms = t - it->reqs[i].submit_ms;
             k++;
         }
-        (void)hipEventDestroy(it->done);
         sl.busy = false;
         it = s->inflight.erase(it);
     }
     *n = k;
-    return CSX_OK;
+    return csx::clear_error();
 }

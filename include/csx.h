@@ -232,8 +232,11 @@ uint64_t csx_arena_used(const csx_arena* arena);
  * max_batch_elements elements), so new arrivals run while earlier batches are
  * still on the device; csx_server_poll retires finished batches without
  * blocking and reports, per request, the caller's id, the selected count and
- * the latency from submission to that poll.  The results equal csx_compact's
- * on each request alone.  One host thread drives a server.  No counterpart in
+ * the latency from submission to that poll (`capacity` must hold a whole
+ * batch: at least max_batch completions).  A step that fails puts the
+ * requests of the failed batch back at the front of the queue.  The results
+ * equal csx_compact's on each request alone.  One host thread drives a
+ * server; csx_server_destroy waits for in-flight batches.  No counterpart in
  * the reference. */
 typedef struct csx_server csx_server;
 typedef struct {

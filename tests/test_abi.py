@@ -124,3 +124,26 @@ def test_pattern_count_matches_oracle(O):
             assert csx.pattern_count(n, pat, L) == O.mask_count(mask, n)
     with pytest.raises(csx.CsxError):
         csx.pattern_count(10, 1, 33)
+
+
+def test_server_and_arena_argument_errors():
+    """Serving entry points reject bad arguments with a message, before any device work."""
+    import gpu_compressstore2_amd.serving as sv
+
+    L = csx.lib()
+    h = ctypes.c_void_p()
+    cfg = sv.ServerConfig(0, 8, 1 << 20)
+    assert L.csx_server_create(ctypes.c_void_p(ctypes.addressof(cfg)), ctypes.byref(h)) == 1
+    assert "streams" in L.csx_last_error().decode()
+    cfg = sv.ServerConfig(17, 8, 1 << 20)
+    assert L.csx_server_create(ctypes.c_void_p(ctypes.addressof(cfg)), ctypes.byref(h)) == 1
+    assert L.csx_server_create(None, ctypes.byref(h)) == 1
+    assert L.csx_arena_create(0, ctypes.byref(h)) == 1 and "zero" in L.csx_last_error().decode()
+    assert L.csx_arena_alloc(None, 16, ctypes.byref(h)) == 1
+    assert L.csx_arena_free(None, None) == 1
+    assert L.csx_arena_used(None) == 0
+    assert L.csx_server_submit(None, None) == 1
+    assert L.csx_server_step(None, None) == 1
+    n = ctypes.c_uint32()
+    assert L.csx_server_poll(None, None, 0, ctypes.byref(n)) == 1
+    assert L.csx_server_queued(None) == 0 and L.csx_server_inflight(None) == 0
