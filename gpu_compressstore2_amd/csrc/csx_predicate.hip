@@ -121,7 +121,10 @@ static hipError_t launch_pred_t(const void* in, uint64_t n, const PredSpec& p, u
     const uint64_t chunks = (n + kPredChunk - 1) / kPredChunk;
     uint64_t g = (chunks + kPredBlock / kWave - 1) / (kPredBlock / kWave);
     if (g < 1) g = 1;
-    if (g > 4096) g = 4096;  // 16 blocks per CU; waves loop over the rest
+    // one chunk per wave up to 2^28 elements (one-pass grids stream fastest on
+    // this HBM); 1-byte columns are ALU-bound and do best with 4 chunks per wave
+    const uint64_t cap = sizeof(T) == 1 ? 16384 : 65536;
+    if (g > cap) g = cap;
     const bool aligned = ((uintptr_t)in & 15) == 0;
     auto x = static_cast<const T*>(in);
 #define CSX_PRED_LAUNCH(ORD, AL) \
