@@ -122,8 +122,10 @@ static hipError_t launch_pred_t(const void* in, uint64_t n, const PredSpec& p, u
     uint64_t g = (chunks + kPredBlock / kWave - 1) / (kPredBlock / kWave);
     if (g < 1) g = 1;
     // one chunk per wave up to 2^28 elements (one-pass grids stream fastest on
-    // this HBM); 1-byte columns are ALU-bound and do best with 4 chunks per wave
-    const uint64_t cap = sizeof(T) == 1 ? 16384 : 65536;
+    // this HBM); 1-byte columns are ALU-bound and do best with 4 chunks per wave.
+    // A running total costs one same-address atomic per block, which the L2
+    // serialises (~16 ns each), so counting grids stay at 4096 blocks.
+    const uint64_t cap = total ? 4096 : sizeof(T) == 1 ? 16384 : 65536;
     if (g > cap) g = cap;
     const bool aligned = ((uintptr_t)in & 15) == 0;
     auto x = static_cast<const T*>(in);
