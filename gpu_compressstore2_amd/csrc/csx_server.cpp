@@ -240,7 +240,7 @@ extern "C" uint32_t csx_server_inflight(const csx_server* s) { return s ? (uint3
 // the oldest queued request's width and every queued request of that width,
 // up to max_batch requests and max_batch_elements elements.  A failed launch
 // puts the batch's requests back at the front of the queue.
-static csx_status launch_batch(csx_server* s, Slot& sl, const Batch& b)
+static csx_status launch_batch(csx_server* s, Slot& sl, const Batch& b, uint64_t elems)
 {
     hipError_t e = hipSuccess;
     for (size_t i = 0; i < b.reqs.size() && e == hipSuccess; i++) {
@@ -251,8 +251,9 @@ static csx_status launch_batch(csx_server* s, Slot& sl, const Batch& b)
     const uint32_t nseg = (uint32_t)b.reqs.size();
     if (e == hipSuccess) e = hipMemcpyAsync(sl.d_table, sl.h_table, sizeof(csx_segment) * nseg, hipMemcpyHostToDevice, sl.stream);
     if (e != hipSuccess) return csx::set_error(CSX_ERR_HIP, "server: batch table upload: %s", hipGetErrorString(e));
-    const csx_status st = csx_compact_batch(sl.d_table, nseg, s->cfg.max_batch_elements, b.width, CSX_MSB_FIRST, sl.d_counts, sl.d_ws, sl.ws_bytes,
-                                            sl.stream);
+    // the batch's exact element total bounds the grids (workgroups past the
+    // last tile only exit, but each is still dispatched)
+    const csx_status st = csx_compact_batch(sl.d_table, nseg, elems, b.width, CSX_MSB_FIRST, sl.d_counts, sl.d_ws, sl.ws_bytes, sl.stream);
     if (st != CSX_OK) return st;
     e = hipMemcpyAsync(sl.h_counts, sl.d_counts, sizeof(uint64_t) * nseg, hipMemcpyDeviceToHost, sl.stream);
     if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
@@ -289,7 +290,7 @@ extern "C" csx_status csx_server_step(csx_server* s, uint32_t* launched)
                 ++it;
             }
         }
-        const csx_status st = launch_batch(s, sl, b);
+        const csx_status st = launch_batch(s, sl, b, elems);
         if (st != CSX_OK) {
             s->queue.insert(s->queue.begin(), b.reqs.begin(), b.reqs.end());
             if (launched) *launched = nl;
