@@ -198,8 +198,11 @@ csx_status csx_compact_streams(const void* d_in, const uint8_t* d_mask, void* d_
  * Sizes are read on the device, so the call launches four kernels with no
  * host round trip and can be captured into a graph once and replayed after
  * rewriting the table.  max_total_elements bounds the sum of n (it sizes the
- * grids and the workspace); a table exceeding it sets the workspace error
- * word (csx_workspace_error) and leaves the excess uncompacted.  No
+ * grids and the workspace, so keep it tight: the write kernel dispatches one
+ * workgroup per tile of the bound, and those past the real tiles only exit);
+ * a table exceeding it sets the workspace error word (csx_workspace_error)
+ * and leaves the excess uncompacted.  A workspace sized for a larger bound
+ * serves any smaller one.  No
  * counterpart in the reference beyond its two-stream split
  * (src/benchmarks.cuh:195-293); MSB- or LSB-first masks, 1/2/4/8-byte
  * elements, all columns of one width. */
