@@ -243,10 +243,14 @@ extern "C" uint32_t csx_server_inflight(const csx_server* s) { return s ? (uint3
 static csx_status launch_batch(csx_server* s, Slot& sl, const Batch& b, uint64_t elems)
 {
     hipError_t e = hipSuccess;
+    hipEvent_t waited = nullptr;  // requests admitted together usually share one event
     for (size_t i = 0; i < b.reqs.size() && e == hipSuccess; i++) {
         const csx_request& r = b.reqs[i].req;
         sl.h_table[i] = csx_segment{r.in, r.mask, r.out, r.n};
-        if (r.ready) e = hipStreamWaitEvent(sl.stream, r.ready, 0);
+        if (r.ready && r.ready != waited) {
+            e = hipStreamWaitEvent(sl.stream, r.ready, 0);
+            waited = r.ready;
+        }
     }
     const uint32_t nseg = (uint32_t)b.reqs.size();
     if (e == hipSuccess) e = hipMemcpyAsync(sl.d_table, sl.h_table, sizeof(csx_segment) * nseg, hipMemcpyHostToDevice, sl.stream);
