@@ -600,14 +600,8 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
         auto counts = reinterpret_cast<uint32_t*>(base + p.counts_off);
         auto offsets = reinterpret_cast<uint64_t*>(base + p.offsets_off);
         const uint32_t log2c = log2u(chunk_len);
-        if (log2c >= 7 && log2c <= 13) {
-            // popcount and scan in one pass
-            CSX_HIP(csx::launch_popc_scan(d_mask, n, log2c, bit_order, offsets, count_dst, hdr, status, stream), "csx_compact popc+scan");
-        }
-        else {
-            CSX_HIP(csx::launch_popc(d_mask, counts, n, log2c, bit_order, stream, hdr, status, p.scan_tiles), "csx_compact popc");
-            CSX_HIP(csx::launch_scan(counts, offsets, p.chunks, count_dst, hdr, status, stream, true), "csx_compact scan");
-        }
+        CSX_HIP(csx::launch_popc(d_mask, counts, n, log2c, bit_order, stream, hdr, status, p.scan_tiles), "csx_compact popc");
+        CSX_HIP(csx::launch_scan(counts, offsets, p.chunks, count_dst, hdr, status, stream, true), "csx_compact scan");
         csx::TileArgs a{};
         a.in = in;
         a.mask = d_mask;

@@ -42,30 +42,6 @@ hipError_t launch_popc(const uint8_t* mask, uint32_t* counts, uint64_t n, uint32
     return hipGetLastError();
 }
 
-hipError_t launch_popc_scan(const uint8_t* mask, uint64_t n, uint32_t log2c, int order, uint64_t* offsets, uint64_t* total, WsHeader* hdr,
-                            uint64_t* status, hipStream_t s)
-{
-    if (log2c < 7 || log2c > 13) return hipErrorInvalidValue;
-    const uint64_t tiles = (n + kPsElems - 1) / kPsElems;
-    hipError_t e = launch_lookback_init(hdr, status, tiles, s);
-    if (e != hipSuccess || tiles == 0) return e;
-    const bool aligned = ((uintptr_t)mask & 15) == 0;
-    const unsigned g = (unsigned)tiles;
-    if (order == kBitMsb) {
-        if (aligned)
-            k_popc_scan<kBitMsb, true><<<g, kPsBlock, 0, s>>>(mask, n, log2c, offsets, total, hdr, status);
-        else
-            k_popc_scan<kBitMsb, false><<<g, kPsBlock, 0, s>>>(mask, n, log2c, offsets, total, hdr, status);
-    }
-    else {
-        if (aligned)
-            k_popc_scan<kBitLsb, true><<<g, kPsBlock, 0, s>>>(mask, n, log2c, offsets, total, hdr, status);
-        else
-            k_popc_scan<kBitLsb, false><<<g, kPsBlock, 0, s>>>(mask, n, log2c, offsets, total, hdr, status);
-    }
-    return hipGetLastError();
-}
-
 hipError_t launch_lookback_init(WsHeader* hdr, uint64_t* status, uint64_t tiles, hipStream_t s)
 {
     k_lookback_init<<<grid_for(tiles, 256, 1024), 256, 0, s>>>(hdr, status, tiles);

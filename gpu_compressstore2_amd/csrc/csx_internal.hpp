@@ -19,11 +19,6 @@ hipError_t launch_popc(const uint8_t* mask, uint32_t* counts, uint64_t n, uint32
                        uint64_t* status = nullptr, uint64_t init_tiles = 0);
 inline uint64_t scan_tiles(uint64_t m) { return (m + kScanTileElems - 1) / kScanTileElems; }
 hipError_t launch_lookback_init(WsHeader* hdr, uint64_t* status, uint64_t tiles, hipStream_t s);
-// phases 1 + 2 in one kernel (after a look-back state reset): offsets and
-// total of chunk lengths 2^7 .. 2^13; the status array needs
-// ceil(n / 262144) words
-hipError_t launch_popc_scan(const uint8_t* mask, uint64_t n, uint32_t log2c, int order, uint64_t* offsets, uint64_t* total, WsHeader* hdr,
-                            uint64_t* status, hipStream_t s);
 // initialized: the look-back state was already zeroed (by launch_popc)
 hipError_t launch_scan(const uint32_t* counts, uint64_t* offsets, uint64_t m, uint64_t* total, WsHeader* hdr, uint64_t* status, hipStream_t s,
                        bool initialized = false);

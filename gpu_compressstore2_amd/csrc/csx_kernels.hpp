@@ -211,93 +211,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const uint32_t* counts, Out
 }
 
 // ======================================================================
-// phases 1 + 2 in one pass, for chunk lengths 128 .. 8192: a workgroup counts
-// the chunks of 32 KiB of mask (262144 elements; a 16-byte vector lies in one
-// chunk, a chunk spans C/128 <= 64 lanes of one load) and scans them; the
-// tiles' running total continues across workgroups by a decoupled look-back
-// on ticketed tile ids.  The look-back state must be zeroed beforehand
-// (k_lookback_init).
-// ======================================================================
-constexpr int kPsBlock = 256;
-constexpr int kPsVecs = 8;                                      // mask vectors per thread
-constexpr uint64_t kPsElems = (uint64_t)kPsBlock * kPsVecs * 128;  // elements per tile
-constexpr int kPsMaxChunks = (int)(kPsElems / 128);             // chunks per tile at C = 128
-
-template <int ORDER, bool ALIGNED>
-__global__ __launch_bounds__(kPsBlock) void k_popc_scan(const uint8_t* __restrict__ mask, uint64_t n, uint32_t log2c, uint64_t* __restrict__ offsets,
-                                                        uint64_t* total, WsHeader* hdr, uint64_t* status)
-{
-    __shared__ uint32_t cnt[kPsMaxChunks];
-    __shared__ uint64_t s_scr[kPsBlock / kWave];
-    __shared__ uint64_t s_tile;
-    __shared__ uint64_t s_prefix;
-    const uint32_t t = threadIdx.x;
-    if (t == 0) s_tile = atomicAdd(&hdr->ticket, 1u);
-    __syncthreads();
-    const uint64_t tile = s_tile;
-    const uint64_t ntiles = (n + kPsElems - 1) / kPsElems;
-    if (tile >= ntiles) return;  // uniform
-    const uint64_t nbytes = (n + 7) / 8, nvec = (nbytes + 15) / 16;
-    const uint32_t lanes = 1u << (log2c - 7);               // lanes per chunk in one load
-    const uint32_t cw = (uint32_t)(kPsElems >> log2c);      // chunks per tile
-    const uint64_t chunks = (n + (1ull << log2c) - 1) >> log2c;
-
-    // counts: all loads first, then popcounts reduced over each chunk's lanes
-    u32x4 x[kPsVecs];
-#pragma unroll
-    for (int j = 0; j < kPsVecs; j++) {
-        const uint64_t v = tile * (kPsBlock * kPsVecs) + (uint64_t)j * kPsBlock + t;
-        x[j] = v < nvec ? load_mask16<ALIGNED>(mask, v * 16, nbytes) : u32x4{0, 0, 0, 0};
-    }
-#pragma unroll
-    for (int j = 0; j < kPsVecs; j++) {
-        const uint64_t v = tile * (kPsBlock * kPsVecs) + (uint64_t)j * kPsBlock + t;
-        uint32_t w[4] = {to_lsb_word<ORDER>(x[j].x), to_lsb_word<ORDER>(x[j].y), to_lsb_word<ORDER>(x[j].z), to_lsb_word<ORDER>(x[j].w)};
-        const uint64_t e0 = v * 128;
-        if (e0 + 128 > n) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const uint64_t b = e0 + 32 * k;
-                w[k] = b >= n ? 0u : low_bits(w[k], (uint32_t)(n - b));
-            }
-        }
-        uint32_t c = popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
-        for (uint32_t o = 1; o < lanes; o <<= 1) c += __shfl_xor(c, o, kWave);
-        const uint32_t local = (uint32_t)j * kPsBlock + t;  // vector index within the tile
-        if ((local & (lanes - 1)) == 0) cnt[local >> (log2c - 7)] = c;
-    }
-    __syncthreads();
-
-    // scan of the tile's chunk counts: thread t owns `per` consecutive chunks
-    const uint32_t per = (cw + kPsBlock - 1) / kPsBlock;
-    const uint32_t i0 = t * per;
-    uint64_t sum = 0;
-    for (uint32_t i = i0; i < i0 + per && i < cw; i++) sum += cnt[i];
-    uint64_t tile_total;
-    const uint64_t excl = block_exclusive_sum<kPsBlock>(sum, s_scr, &tile_total);
-    if (t < kWave) {
-        uint64_t pre = 0;
-        if (tile == 0) {
-            if (t == 0) publish_status(status, kFlagPrefix | tile_total);
-        }
-        else {
-            if (t == 0) publish_status(status + tile, kFlagAggregate | tile_total);
-            pre = lookback_exclusive(status, tile, &hdr->error);
-            if (t == 0) publish_status(status + tile, kFlagPrefix | (pre + tile_total));
-        }
-        if (t == 0) s_prefix = pre;
-    }
-    __syncthreads();
-    uint64_t run = s_prefix + excl;
-    for (uint32_t i = i0; i < i0 + per && i < cw; i++) {
-        const uint64_t c = tile * cw + i;
-        if (c < chunks) offsets[c] = run;
-        run += cnt[i];
-    }
-    if (tile == ntiles - 1 && t == 0 && total) *total = s_prefix + tile_total;
-}
-
-// ======================================================================
 // phase 3 / fused: the compaction tile engine
 // ======================================================================
 //
