@@ -553,7 +553,10 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
     if (n > 0 && (!d_in || !d_mask || !d_out)) return fail(CSX_ERR_INVALID, "null input, mask or output pointer");
     if (((uintptr_t)d_in % elem_size) || ((uintptr_t)d_out % elem_size))
         return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
-    if (!launch_fits(p.fused_tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)p.fused_tiles);
+    // the writeout's grid is one workgroup per tile of the chosen size
+    const uint64_t wtiles = (n + 256ull * vpt * (16 / elem_size) - 1) / (256ull * vpt * (16 / elem_size));
+    if (!launch_fits(p.fused_tiles) || !launch_fits(wtiles))
+        return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)std::max(p.fused_tiles, wtiles));
 
     auto base = static_cast<uint8_t*>(d_ws);
     auto hdr = reinterpret_cast<csx::WsHeader*>(base);

@@ -81,3 +81,14 @@ def test_config_checks_before_device_work():
     assert L.csx_autotune(4, 0, 0.5, 3, ctypes.byref(c), ctypes.byref(ms), None) == 1
     assert L.csx_autotune(4, 100, 1.5, 3, ctypes.byref(c), ctypes.byref(ms), None) == 1
     assert csx.auto_workspace_size(1 << 20, 4) == csx.workspace_size(1 << 20, 4, 256)
+
+
+def test_tile_grid_bound_checked_for_the_chosen_tile():
+    """2^38 one-byte elements fit one launch with 8-vector tiles but not with
+    2-vector tiles (2^25 workgroups of 256 threads); the call refuses before
+    any device work."""
+    L = csx.lib()
+    c = csx.Config(csx.PIPELINE_THREE_PHASE, 1 << 17, 2, 0)
+    fake = ctypes.c_void_p(1 << 20)
+    st = L.csx_compact_config(fake, fake, fake, 1 << 38, 1, 0, ctypes.byref(c), None, None, fake, 2**63, None)
+    assert st == 4 and "exceed one launch" in L.csx_last_error().decode()
