@@ -4,14 +4,17 @@
 
 Workload: 2^28 four-byte elements (float32 uniform in [0,1), moved as opaque
 32-bit words) with packed MSB-first Bernoulli masks at 1 %, 50 % and 99 %
-selectivity.  A step is one csx_compact call (three-phase pipeline)
-over one input set, inputs already resident in HBM; three input sets rotate
-so the 256 MB Infinity Cache cannot serve a step from the previous one.
+selectivity.  A step is one compaction call over one input set, inputs already
+resident in HBM; three input sets rotate so the 256 MB Infinity Cache cannot
+serve a step from the previous one.
 
 Metric (effective GB/s) = (data bytes read + mask bytes read + selected bytes
-written) / time = (4N + N/8 + 4k) / t.  `value` is the 50 % point; the other
-selectivities, the two-pass and fused pipelines and a plain device copy are reported
-under "sweep".  "roofline" prices the dominant kernel (the staged writeout)
+written) / time = (4N + N/8 + 4k) / t.  `value` is the 50 % point of
+csx_compact_auto (the configuration the autotuner measured for this width,
+size and selectivity: the shipped tuning table); every selectivity's default
+three-phase, two-pass, fused and tuned forms, rocPRIM's select, the batched
+4096-column workload, the predicate path, the serving loop and a plain device
+copy are reported under "sweep".  "roofline" prices the dominant kernel (the staged writeout)
 against the 8 TB/s HBM peak with its duration from HIP events recorded on its
 own stream; "cpu_baseline" times the reference's CPU compaction
 (generate_validation, compiled from its sources into oracle/_ref) on the host.
@@ -33,6 +36,8 @@ import gpu_compressstore2_amd as csx  # noqa: E402
 from gpu_compressstore2_amd import baseline  # noqa: E402
 
 METRIC = "effective GB/s (data+mask+out) & % HBM peak, 2^28 u32 at 1/50/99% sel"
+# counter summaries (HBM bytes per launch) of the 50 % writeout, by tile vectors per thread
+TRAFFIC_FILES = {8: "traffic_writeout_2p28_u32_p50.json", 2: "traffic_writeout_vpt2_2p28_u32_p50.json"}
 HBM_PEAK_GBS = 8000.0
 ROTATE = 3
 
@@ -253,15 +258,24 @@ def main():
                 print(f"[bench] MISMATCH {name} p={p}", file=sys.stderr)
                 sys.exit(1)
             del expect
-            if name == "three_phase" and p == 0.5:
-                main_result = (gbs, ms / K, kern_avg, step_bytes)
-        # the tuned configuration for this width, size and selectivity
+        # the tuned configuration for this width, size and selectivity (the headline at 0.5)
         ws_auto = csx.alloc_workspace(csx.auto_workspace_size(n, w))
-        ms = time_steps(lambda i: csx.compact_auto(datas[i % ROTATE], masks[i % ROTATE], p, n=n, out=out, count=count, workspace=ws_auto,
-                                                   host_count=False), K, W)
+
+        def auto_step(i):
+            csx.compact_auto(datas[i % ROTATE], masks[i % ROTATE], p, n=n, out=out, count=count, workspace=ws_auto, host_count=False)
+
+        ms = time_steps(auto_step, K, W)
+        csx.profile_begin()
+        for i in range(K):
+            auto_step(i)
+        kern_ms, launches = csx.profile_end()
+        kern_avg = kern_ms / max(launches, 1)
         gbs = K * step_bytes / (ms * 1e-3) / 1e9
+        cfg = csx.tune_lookup(w, n, p).as_dict()
         sweep[f"auto@{p}"] = {"GB/s": round(gbs, 1), "pct_peak": round(100 * gbs / HBM_PEAK_GBS, 2), "ms_per_step": round(ms / K, 5),
-                              "config": csx.tune_lookup(w, n, p).as_dict()}
+                              "kernel_ms": round(kern_avg, 5), "config": cfg}
+        if p == 0.5:
+            main_result = (gbs, ms / K, kern_avg, step_bytes, cfg)
         print(f"[bench] auto p={p}: {gbs:.1f} GB/s ({ms / K:.4f} ms/step) {sweep[f'auto@{p}']['config']}", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         if int(count.item()) != ks[(K - 1) % ROTATE]:
@@ -307,11 +321,12 @@ def main():
     copy_ms = time_steps(lambda i: csx.copy(out, datas[i % ROTATE]), K, W)
     sweep["copy_4N"] = {"GB/s": round(K * 2 * 4 * n / (copy_ms * 1e-3) / 1e9, 1), "ms_per_step": round(copy_ms / K, 5)}
 
-    gbs, ms_step, kern_avg, step_bytes = main_result
+    gbs, ms_step, kern_avg, step_bytes, cfg = main_result
     achieved = step_bytes / (kern_avg * 1e-3) / 1e9
+    # HBM bytes per launch of this tile size's writeout, from the committed counter summaries
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_writeout_2p28_u32_p50.json")
-    if os.path.exists(tpath):
+    tpath = os.path.join(ROOT, "profiles", TRAFFIC_FILES.get(cfg["vpt"], ""))
+    if os.path.isfile(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {
@@ -321,7 +336,7 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "kernel": "csx::k_tile<4,MSB,aligned,writeout,staged> (phase 3)",
+        "kernel": f"csx::k_tile<4,MSB,aligned,writeout,staged,vpt={cfg['vpt']}> (phase 3)",
         "algorithmic_bytes_per_launch": int(step_bytes),
     }
 
@@ -343,11 +358,14 @@ def main():
         "dtype": "u32",
         "data": "synthetic: float32 uniform [0,1) moved as u32 words, seeded torch generators, MSB-first Bernoulli masks, 3 rotating input sets",
         "config": {
-            "workload": "single-column compaction of 2^28 4-byte elements, Bernoulli(0.5) mask, three-phase pipeline (popc, look-back scan, staged writeout), chunk 1024",
+            "workload": "single-column compaction of 2^28 4-byte elements, Bernoulli(0.5) mask, tuned configuration (csx_compact_auto: "
+                        f"three-phase popc, look-back scan, staged writeout; chunk {cfg['chunk_len']}, {cfg['vpt']} vectors per thread)",
             "n": n,
             "elem_bytes": w,
             "selectivity": 0.5,
-            "chunk_len": args.chunk,
+            "chunk_len": cfg["chunk_len"],
+            "tile_vectors_per_thread": cfg["vpt"],
+            "default_config_GB/s": sweep["three_phase@0.5"]["GB/s"],
             "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
         },
         "roofline": roofline,
