@@ -276,7 +276,7 @@ def main():
                               "kernel_ms": round(kern_avg, 5), "config": cfg}
         if p == 0.5:
             main_result = (gbs, ms / K, kern_avg, step_bytes, cfg)
-        print(f"[bench] auto p={p}: {gbs:.1f} GB/s ({ms / K:.4f} ms/step) {sweep[f'auto@{p}']['config']}", file=sys.stderr, flush=True)
+        print(f"[bench] auto p={p}: {gbs:.1f} GB/s ({ms / K:.4f} ms/step, kernel {kern_avg:.4f} ms) {cfg}", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         if int(count.item()) != ks[(K - 1) % ROTATE]:
             print(f"[bench] auto count mismatch at p={p}", file=sys.stderr)
