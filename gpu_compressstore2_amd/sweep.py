@@ -1,0 +1,102 @@
+"""Per-phase sweep of the single-column compaction (2^28 float32 by default).
+
+    python -m gpu_compressstore2_amd.sweep [--log2n 28] [--reps 10] [--json FILE]
+
+Masks: Bernoulli at 1, 10, 50, 90 and 99 % selectivity, and alternating
+1-runs / 0-runs with geometric lengths of mean 256 (50 %).  Chunk lengths 256,
+1024 and 4096.  For every writeout variant (staged, direct, thread per chunk)
+the three phases -- popcount, scan, writeout -- are timed separately with HIP
+events on the launching stream (median of `reps`), the way the reference's
+benchmarks report time_popc / time_pss / time_proc (src/benchmarks.cuh:10-52);
+the whole-call forms (three-phase, two-pass, fused, tuned) are timed as one.
+Every configuration's count is checked against the mask's popcount.  One CSV
+line per configuration; GB/s counts n*4 + n/8 + k*4 bytes.
+"""
+import argparse
+import json
+import sys
+
+import torch
+
+import gpu_compressstore2_amd as csx
+
+VARIANTS = {"staged": csx.WRITEOUT_STAGED, "direct": csx.WRITEOUT_DIRECT, "thread_per_chunk": csx.WRITEOUT_THREAD_PER_CHUNK}
+PIPELINES = {"three_phase": csx.PIPELINE_THREE_PHASE, "two_pass": csx.PIPELINE_TWO_PASS, "fused": csx.PIPELINE_FUSED}
+
+
+def median_ms(fn, reps):
+    fn()
+    times = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        times.append(a.elapsed_time(b))
+    times.sort()
+    return times[len(times) // 2]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--log2n", type=int, default=28)
+    ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--json", default=None, help="also write the rows as JSON")
+    args = ap.parse_args()
+    n = 1 << args.log2n
+    g = torch.Generator(device="cuda").manual_seed(11)
+    data = torch.rand(n, device="cuda", generator=g)
+    out = torch.empty_like(data)
+    masks = {f"bernoulli_{p}": csx.gen_mask_bernoulli(n, p, seed=7) for p in (0.01, 0.1, 0.5, 0.9, 0.99)}
+    masks["runs_256"] = csx.gen_mask_runs(n, 256, seed=7)
+    count = torch.empty(1, dtype=torch.int64, device="cuda")
+    rows = []
+    print("mask;chunk_len;form;time_popc_ms;time_scan_ms;time_writeout_ms;time_total_ms;selected;gbps;pct_peak", flush=True)
+
+    def emit(mname, chunk, form, tp, ts, tw, total, k):
+        gbps = (4 * n + n / 8 + 4 * k) / (total * 1e-3) / 1e9
+        row = {"mask": mname, "chunk_len": chunk, "form": form, "popc_ms": tp, "scan_ms": ts, "writeout_ms": tw, "total_ms": round(total, 5),
+               "selected": k, "GB/s": round(gbps, 1), "pct_peak": round(gbps / 80.0, 2)}
+        rows.append(row)
+        f = lambda v: "" if v is None else f"{v:.5f}"
+        print(f"{mname};{chunk};{form};{f(tp)};{f(ts)};{f(tw)};{total:.5f};{k};{gbps:.1f};{gbps / 80.0:.2f}", flush=True)
+
+    for mname, mask in masks.items():
+        for chunk in (256, 1024, 4096):
+            counts = csx.popc(mask, n, chunk)
+            offsets, total = csx.scan(counts)
+            torch.cuda.synchronize()
+            k = int(total.item())
+            expect = int(counts.to(torch.int64).sum().item())
+            if k != expect:
+                print(f"count mismatch {mname} chunk {chunk}: scan {k}, popcount {expect}", file=sys.stderr)
+                sys.exit(1)
+            scan_ws = csx.alloc_workspace(csx.scan_workspace_size(counts.numel()))
+            tp = median_ms(lambda: csx.popc(mask, n, chunk, counts=counts), args.reps)
+            ts = median_ms(lambda: csx.scan(counts, offsets=offsets, total=total, workspace=scan_ws), args.reps)
+            for vname, v in VARIANTS.items():
+                reps = args.reps if v != csx.WRITEOUT_THREAD_PER_CHUNK else max(2, args.reps // 5)
+                tw = median_ms(lambda: csx.writeout(data, mask, offsets, n, chunk, variant=v, counts=counts, out=out), reps)
+                emit(mname, chunk, f"phases_{vname}", round(tp, 5), round(ts, 5), round(tw, 5), tp + ts + tw, k)
+            ws = csx.alloc_workspace(csx.workspace_size(n, 4, chunk))
+            for pname, pl in PIPELINES.items():
+                tt = median_ms(lambda: csx.compact(data, mask, n=n, chunk_len=chunk, pipeline=pl, out=out, count=count, workspace=ws, host_count=False),
+                               args.reps)
+                if int(count.item()) != k:
+                    print(f"count mismatch {mname} chunk {chunk} {pname}", file=sys.stderr)
+                    sys.exit(1)
+                emit(mname, chunk, pname, None, None, None, tt, k)
+            del ws, scan_ws
+        sel = k / n
+        wsa = csx.alloc_workspace(csx.auto_workspace_size(n, 4))
+        ta = median_ms(lambda: csx.compact_auto(data, mask, sel, n=n, out=out, count=count, workspace=wsa, host_count=False), args.reps)
+        emit(mname, 0, "tuned", None, None, None, ta, k)
+        del wsa
+    if args.json:
+        with open(args.json, "w") as f:
+            json.dump(rows, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
