@@ -64,16 +64,23 @@ __global__ __launch_bounds__(256) void k_batch_plan(const Segment* segs, uint32_
     }
 }
 
-// segment s with tile_prefix[s] <= g < tile_prefix[s + 1]
+// Segment s with tile_prefix[s] <= g < tile_prefix[s + 1] (the last such s:
+// empty segments repeat a prefix).  Called by all 64 lanes of a wave with the
+// same g: each round the lanes probe 64 evenly spaced points of the interval
+// and keep the stretch after the last probe that is <= g, so 4096 segments
+// take two dependent loads instead of twelve.  Lane 0 probes lo, whose prefix
+// is <= g throughout, so the ballot is never empty.
 __device__ __forceinline__ uint32_t find_segment(const uint64_t* tile_prefix, uint32_t nseg, uint64_t g)
 {
+    const uint32_t lane = lane_id();
     uint32_t lo = 0, hi = nseg;  // answer in [lo, hi)
     while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (tile_prefix[mid] <= g)
-            lo = mid;
-        else
-            hi = mid;
+        const uint32_t step = (hi - lo + kWave - 1) / kWave;
+        const uint32_t p = lo + lane * step;
+        const uint64_t le = __ballot(p < hi && tile_prefix[p] <= g);
+        const uint32_t last = 63u - (uint32_t)__builtin_clzll(le);
+        lo = lo + last * step;
+        hi = hi < lo + step ? hi : lo + step;
     }
     return lo;
 }
@@ -122,7 +129,10 @@ __global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32
     __shared__ uint32_t s_seg;
     const uint64_t g = blockIdx.x;
     if (g >= tile_prefix[nseg]) return;  // surplus of the host's bound: uniform exit
-    if (threadIdx.x == 0) s_seg = find_segment(tile_prefix, nseg, g);
+    if (threadIdx.x < kWave) {
+        const uint32_t s0 = find_segment(tile_prefix, nseg, g);  // wave 0 together
+        if (threadIdx.x == 0) s_seg = s0;
+    }
     __syncthreads();
     const uint32_t s = s_seg;
     const Segment seg = segs[s];
