@@ -86,8 +86,7 @@ __device__ __forceinline__ uint32_t find_segment(const uint64_t* tile_prefix, ui
 }
 
 template <int W, int ORDER, int VPT>
-__global__ __launch_bounds__(256) void k_batch_counts(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, uint32_t* tcounts,
-                                                      uint32_t* tseg)
+__global__ __launch_bounds__(256) void k_batch_counts(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, uint32_t* tcounts)
 {
     constexpr uint64_t TE = TileCfg<W, VPT>::TE;
     constexpr uint32_t VEC_PER_TILE = (uint32_t)(TE / 128);
@@ -117,22 +116,25 @@ __global__ __launch_bounds__(256) void k_batch_counts(const Segment* segs, uint3
             c += popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
         }
         c = wave_sum(c);
-        if (lane == 0) {
-            tcounts[g] = c;
-            tseg[g] = s;  // the write kernel's segment lookup
-        }
+        if (lane == 0) tcounts[g] = c;
     }
 }
 
 template <int W, int ORDER, int VPT>
-__global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, const uint32_t* tseg,
-                                                     const uint64_t* goff, uint64_t* counts)
+__global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, const uint64_t* goff,
+                                                     uint64_t* counts)
 {
     using C = TileCfg<W, VPT>;
     __shared__ TileShared<C> sh;
+    __shared__ uint32_t s_seg;
     const uint64_t g = blockIdx.x;
     if (g >= tile_prefix[nseg]) return;  // surplus of the host's bound: uniform exit
-    const uint32_t s = tseg[g];          // recorded by the count kernel
+    if (threadIdx.x < kWave) {
+        const uint32_t s0 = find_segment(tile_prefix, nseg, g);  // wave 0 together
+        if (threadIdx.x == 0) s_seg = s0;
+    }
+    __syncthreads();
+    const uint32_t s = s_seg;
     const Segment seg = segs[s];
     const uint64_t first = tile_prefix[s];
     TileArgs a{};
@@ -157,7 +159,7 @@ uint64_t batch_max_tiles(uint32_t w, uint32_t nseg, uint64_t max_total, uint32_t
 
 template <int W, int ORDER, int VPT>
 static hipError_t batch_impl(const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix, uint32_t* tcounts,
-                             uint32_t* tseg, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks, hipStream_t s)
+                             uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks, hipStream_t s)
 {
     const uint64_t status_words = (max_tiles + kScanTile - 1) / kScanTile;
     k_batch_plan<<<1, 256, 0, s>>>(segs, nseg, TileCfg<W, VPT>::TE, max_tiles, tile_prefix, counts, hdr, status, status_words);
@@ -165,45 +167,44 @@ static hipError_t batch_impl(const Segment* segs, uint32_t nseg, uint64_t max_ti
     if (e != hipSuccess || max_tiles == 0) return e;
     uint64_t g1 = (max_tiles + 3) / 4;
     if (g1 > count_blocks) g1 = count_blocks;
-    k_batch_counts<W, ORDER, VPT><<<(unsigned)g1, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts, tseg);
+    k_batch_counts<W, ORDER, VPT><<<(unsigned)g1, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_scan<uint64_t><<<(unsigned)status_words, kScanBlock, 0, s>>>(tcounts, goff, max_tiles, nullptr, hdr, status, tile_prefix + nseg);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_batch_write<W, ORDER, VPT><<<(unsigned)max_tiles, 256, 0, s>>>(segs, nseg, tile_prefix, tseg, goff, counts);
+    k_batch_write<W, ORDER, VPT><<<(unsigned)max_tiles, 256, 0, s>>>(segs, nseg, tile_prefix, goff, counts);
     return hipGetLastError();
 }
 
 template <int W, int VPT>
 static hipError_t batch_order(int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
-                              uint32_t* tcounts, uint32_t* tseg, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks, hipStream_t s)
+                              uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks, hipStream_t s)
 {
-    return order == kBitMsb ? batch_impl<W, kBitMsb, VPT>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s)
-                            : batch_impl<W, kBitLsb, VPT>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
+    return order == kBitMsb ? batch_impl<W, kBitMsb, VPT>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s)
+                            : batch_impl<W, kBitLsb, VPT>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
 }
 
 template <int W>
 static hipError_t batch_vpt(uint32_t vpt, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts,
-                            uint64_t* tile_prefix, uint32_t* tcounts, uint32_t* tseg, uint64_t* goff, WsHeader* hdr, uint64_t* status,
-                            uint32_t count_blocks, hipStream_t s)
+                            uint64_t* tile_prefix, uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks,
+                            hipStream_t s)
 {
     switch (vpt) {
-        case 2: return batch_order<W, 2>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
-        case 4: return batch_order<W, 4>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
-        case 8: return batch_order<W, 8>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
+        case 2: return batch_order<W, 2>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 4: return batch_order<W, 4>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 8: return batch_order<W, 8>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_batch(uint32_t w, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
-                        uint32_t* tcounts, uint32_t* tseg, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s, uint32_t vpt,
-                        uint32_t count_blocks)
+                        uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s, uint32_t vpt, uint32_t count_blocks)
 {
     if (count_blocks < 1) count_blocks = 1;
     switch (w) {
-        case 1: return batch_vpt<1>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
-        case 2: return batch_vpt<2>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
-        case 4: return batch_vpt<4>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
-        default: return batch_vpt<8>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
+        case 1: return batch_vpt<1>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 2: return batch_vpt<2>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 4: return batch_vpt<4>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        default: return batch_vpt<8>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
     }
 }
 
