@@ -3,14 +3,19 @@
 // Requests arrive as a Poisson process (default 20000/s) for a fixed time.
 // Each has an element type drawn uniformly from {uint32, float32, uint64}, a
 // length log-uniform in [2^a, 2^b] (default [2^12, 2^26]) and a Bernoulli
-// mask of selectivity uniform in [0, 1].  A request is admitted when its
-// input, mask and output fit in the HBM budget (a csx_arena, FIFO order);
-// its inputs are then produced on a generator stream and it is submitted to
-// the server, whose batches overlap on a ring of streams.  Reported: request
-// latency from arrival to the poll that saw it finish (p50/p99/max) and the
-// aggregate effective GB/s (input + mask + selected output bytes) of
-// completed requests.  Every k-th request is recomputed alone with
-// csx_compact and compared element for element.
+// mask of selectivity uniform in [0, 1].
+//
+// Two host threads.  The client thread plays the clients: it draws the
+// arrivals, admits them in arrival order when input, mask and output fit in
+// the HBM budget (a csx_arena), produces their inputs on the device on its
+// own stream and hands each admission round (with one ready event) to the
+// server thread.  The server thread drives the csx_server: it submits the
+// handed-over requests, steps (batches launch on a ring of streams) and
+// polls, and recomputes every k-th finished request alone with csx_compact,
+// comparing element for element on the device, asynchronously.  Reported:
+// request latency from arrival to the poll that saw it finish (p50/p99/max)
+// and the aggregate effective GB/s (input + mask + selected output bytes) of
+// completed requests.
 //
 //   csx_serve [-r rate] [-t seconds] [-B budget_GiB] [-a log2min] [-b log2max]
 //             [-S streams] [-m max_batch] [-s seed] [-v verify_every]
@@ -19,6 +24,9 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -134,17 +142,101 @@ int main(int argc, char** argv)
         hk(hipEventCreateWithFlags(&c.done, hipEventDisableTiming), "event");
     }
 
-    std::mt19937_64 rng(seed);
-    std::exponential_distribution<double> gap(rate / 1000.0);  // per ms
-    std::uniform_real_distribution<double> unif(0.0, 1.0);
-    std::deque<Spec> arrivals;  // arrived, not yet admitted
+    // ---- hand-off from the client thread to the server thread ----
+    struct Handoff {
+        std::mutex mu;
+        std::vector<Live> ready;  // admitted, inputs being produced; arrival order
+        bool done = false;        // no more arrivals
+    } hand;
+    std::atomic<uint64_t> produced{0}, waits{0}, peak_used{0};
+    std::atomic<double> host_admit{0};
+    const double t0 = now_ms();
+    const double t_end = t0 + seconds * 1000.0;
+
+    std::thread client([&] {
+        std::mt19937_64 rng(seed);
+        std::exponential_distribution<double> gap(rate / 1000.0);  // per ms
+        std::uniform_real_distribution<double> unif(0.0, 1.0);
+        std::deque<Spec> arrivals;  // arrived, not yet admitted
+        double next_arrival = t0 + gap(rng);
+        uint64_t next_id = 0;
+        double admit_ms = 0;
+        while (true) {
+            const double t = now_ms();
+            while (next_arrival <= t && next_arrival < t_end) {
+                Spec sp{};
+                sp.id = next_id++;
+                sp.arrival_ms = next_arrival;
+                sp.n = (uint64_t)std::exp(std::log((double)(1ull << lmin)) +
+                                          unif(rng) * (std::log((double)(1ull << lmax)) - std::log((double)(1ull << lmin))));
+                const int type = (int)(unif(rng) * 3.0);  // 0 uint32, 1 float32, 2 uint64
+                sp.w = type == 2 ? 8 : 4;
+                sp.kind = type == 1 ? CSX_VALUES_FLOAT : CSX_VALUES_INTEGER;
+                sp.p = unif(rng);
+                arrivals.push_back(sp);
+                next_arrival += gap(rng);
+            }
+            // admission in arrival order while the budget allows
+            ReadyEvent* round = nullptr;
+            std::vector<Live> fresh;
+            while (!arrivals.empty()) {
+                const Spec& sp = arrivals.front();
+                void *in = nullptr, *mask = nullptr, *out = nullptr;
+                bool fits = csx_arena_alloc(arena, sp.n * sp.w, &in) == CSX_OK;
+                if (fits && csx_arena_alloc(arena, (sp.n + 7) / 8, &mask) != CSX_OK) {
+                    csx_arena_free(arena, in);
+                    fits = false;
+                }
+                if (fits && csx_arena_alloc(arena, sp.n * sp.w, &out) != CSX_OK) {
+                    csx_arena_free(arena, in);
+                    csx_arena_free(arena, mask);
+                    fits = false;
+                }
+                if (!fits) {
+                    if (csx_arena_used(arena) == 0) die("admission", "a request is larger than the whole budget");
+                    break;
+                }
+                ck(csx_gen_values(in, sp.n, sp.w, sp.kind, seed * 1000003ull + sp.id, gen), "generate values");
+                ck(csx_gen_mask_bernoulli(static_cast<uint8_t*>(mask), sp.n, sp.p, seed * 7919ull + sp.id, CSX_MSB_FIRST, gen), "generate mask");
+                if (!round) {
+                    round = new ReadyEvent{nullptr, 0};
+                    hk(hipEventCreateWithFlags(&round->ev, hipEventDisableTiming), "event");
+                }
+                round->refs++;
+                fresh.push_back(Live{sp, in, mask, out, round});
+                arrivals.pop_front();
+            }
+            if (round) {
+                hk(hipEventRecord(round->ev, gen), "event record");
+                std::lock_guard<std::mutex> lk(hand.mu);
+                for (Live& L : fresh) hand.ready.push_back(L);
+            }
+            if (!arrivals.empty()) waits++;
+            uint64_t used = csx_arena_used(arena), prev = peak_used.load();
+            while (used > prev && !peak_used.compare_exchange_weak(prev, used)) {
+            }
+            admit_ms += now_ms() - t;
+            if (t >= t_end && arrivals.empty()) break;
+            if (fresh.empty()) {
+                // idle until the next arrival (at most 20 us), or 10 us while waiting for budget
+                const double wait_us = arrivals.empty() ? std::min(20.0, (next_arrival - now_ms()) * 1000.0) : 10.0;
+                if (wait_us >= 2.0) usleep((useconds_t)wait_us);
+            }
+        }
+        produced = next_id;
+        host_admit = admit_ms;
+        std::lock_guard<std::mutex> lk(hand.mu);
+        hand.done = true;
+    });
+
+    // ---- the server thread (this one) ----
     std::unordered_map<uint64_t, Live> live;
     std::vector<double> latencies;
     std::vector<csx_completion> done(4096);
-    uint64_t next_id = 0, admitted = 0, completed = 0, verified = 0, mismatches = 0, waits = 0;
+    std::vector<Live> incoming;
+    uint64_t completed = 0, verified = 0, mismatches = 0;
     double bytes = 0.0, last_done_ms = 0.0;
-    uint64_t peak_used = 0;
-    double host_admit = 0, host_step = 0, host_poll = 0, host_check = 0;
+    double host_step = 0, host_poll = 0, host_check = 0;
     auto release = [&](Live& L) {
         csx_arena_free(arena, L.in);
         csx_arena_free(arena, L.mask);
@@ -160,7 +252,7 @@ int main(int argc, char** argv)
         release(c.req);
         c.busy = false;
     };
-    // retire finished checks; with wait, block for the oldest busy one
+    // retire finished checks; with wait, block for every busy one
     auto sweep_checks = [&](bool wait) {
         for (Check& c : checks) {
             if (!c.busy) continue;
@@ -190,77 +282,29 @@ int main(int argc, char** argv)
         hk(hipMemcpyAsync(c->h_res, c->d_res, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, ver), "check copy");
         hk(hipEventRecord(c->done, ver), "check record");
     };
-    const double t0 = now_ms();
-    double next_arrival = t0 + gap(rng);
-    const double t_end = t0 + seconds * 1000.0;
 
     while (true) {
-        const double t = now_ms();
-        // arrivals up to now
-        while (next_arrival <= t && next_arrival < t_end) {
-            Spec s{};
-            s.id = next_id++;
-            s.arrival_ms = next_arrival;
-            s.n = (uint64_t)std::exp(std::log((double)(1ull << lmin)) + unif(rng) * (std::log((double)(1ull << lmax)) - std::log((double)(1ull << lmin))));
-            const int type = (int)(unif(rng) * 3.0);  // 0 uint32, 1 float32, 2 uint64
-            s.w = type == 2 ? 8 : 4;
-            s.kind = type == 1 ? CSX_VALUES_FLOAT : CSX_VALUES_INTEGER;
-            s.p = unif(rng);
-            arrivals.push_back(s);
-            next_arrival += gap(rng);
+        bool client_done;
+        {
+            std::lock_guard<std::mutex> lk(hand.mu);
+            incoming.swap(hand.ready);
+            client_done = hand.done;
         }
-        // admission in arrival order while the budget allows
-        const double ta = now_ms();
-        ReadyEvent* round = nullptr;
-        std::vector<uint64_t> fresh;
-        while (!arrivals.empty()) {
-            const Spec& s = arrivals.front();
-            void *in = nullptr, *mask = nullptr, *out = nullptr;
-            bool fits = csx_arena_alloc(arena, s.n * s.w, &in) == CSX_OK;
-            if (fits && csx_arena_alloc(arena, (s.n + 7) / 8, &mask) != CSX_OK) {
-                csx_arena_free(arena, in);
-                fits = false;
-            }
-            if (fits && csx_arena_alloc(arena, s.n * s.w, &out) != CSX_OK) {
-                csx_arena_free(arena, in);
-                csx_arena_free(arena, mask);
-                fits = false;
-            }
-            if (!fits) {
-                if (live.empty()) die("admission", "a request is larger than the whole budget");
-                break;
-            }
-            ck(csx_gen_values(in, s.n, s.w, s.kind, seed * 1000003ull + s.id, gen), "generate values");
-            ck(csx_gen_mask_bernoulli(static_cast<uint8_t*>(mask), s.n, s.p, seed * 7919ull + s.id, CSX_MSB_FIRST, gen), "generate mask");
-            if (!round) {
-                round = new ReadyEvent{nullptr, 0};
-                hk(hipEventCreateWithFlags(&round->ev, hipEventDisableTiming), "event");
-            }
-            round->refs++;
-            live[s.id] = Live{s, in, mask, out, round};
-            fresh.push_back(s.id);
-            arrivals.pop_front();
-            admitted++;
+        for (Live& L : incoming) {
+            live[L.spec.id] = L;
+            csx_request r{};
+            r.id = L.spec.id;
+            r.in = L.in;
+            r.mask = static_cast<const uint8_t*>(L.mask);
+            r.out = L.out;
+            r.n = L.spec.n;
+            r.elem_size = L.spec.w;
+            r.ready = L.ready->ev;
+            ck(csx_server_submit(server, &r), "submit");
         }
-        if (round) {
-            hk(hipEventRecord(round->ev, gen), "event record");
-            for (uint64_t id : fresh) {
-                const Live& L = live[id];
-                csx_request r{};
-                r.id = L.spec.id;
-                r.in = L.in;
-                r.mask = static_cast<const uint8_t*>(L.mask);
-                r.out = L.out;
-                r.n = L.spec.n;
-                r.elem_size = L.spec.w;
-                r.ready = round->ev;
-                ck(csx_server_submit(server, &r), "submit");
-            }
-        }
-        if (!arrivals.empty()) waits++;
-        peak_used = std::max(peak_used, csx_arena_used(arena));
+        const bool got = !incoming.empty();
+        incoming.clear();
         const double ts = now_ms();
-        host_admit += ts - ta;
         uint32_t launched = 0, nd = 0;
         ck(csx_server_step(server, &launched), "step");
         const double tq = now_ms();
@@ -281,14 +325,15 @@ int main(int argc, char** argv)
             live.erase(it);
         }
         sweep_checks(false);
-        const double tc = now_ms();
-        host_check += tc - tp;
-        if (tp >= t_end && arrivals.empty() && live.empty()) {
-            sweep_checks(true);
-            break;
+        host_check += now_ms() - tp;
+        if (client_done && live.empty()) {
+            std::lock_guard<std::mutex> lk(hand.mu);
+            if (hand.ready.empty()) break;
         }
-        if (nd == 0 && launched == 0 && arrivals.empty() && next_arrival > tp + 0.05 && tp < t_end) usleep(20);
+        if (!got && nd == 0 && launched == 0) usleep(10);
     }
+    sweep_checks(true);
+    client.join();
     const double wall_ms = last_done_ms - t0;
     std::sort(latencies.begin(), latencies.end());
     auto pct = [&](double q) { return latencies.empty() ? 0.0 : latencies[std::min(latencies.size() - 1, (size_t)(q * latencies.size()))]; };
@@ -298,11 +343,11 @@ int main(int argc, char** argv)
     printf("{\"requests\": %llu, \"completed\": %llu, \"seconds\": %.3f, \"offered_rate\": %.0f, \"achieved_rate\": %.1f, \"GB/s\": %.1f, "
            "\"p50_ms\": %.3f, \"p99_ms\": %.3f, \"max_ms\": %.3f, \"mean_ms\": %.3f, \"verified\": %llu, \"mismatches\": %llu, "
            "\"peak_budget_GiB\": %.2f, \"budget_GiB\": %.1f, \"admission_waits\": %llu, \"streams\": %d, \"max_batch\": %d, "
-           "\"host_ms\": {\"admit\": %.0f, \"step\": %.0f, \"poll\": %.0f, \"check\": %.0f}}\n",
-           (unsigned long long)next_id, (unsigned long long)completed, wall_ms / 1000.0, rate, completed / (wall_ms / 1000.0),
+           "\"host_ms\": {\"client_admit\": %.0f, \"step\": %.0f, \"poll\": %.0f, \"check\": %.0f}}\n",
+           (unsigned long long)produced.load(), (unsigned long long)completed, wall_ms / 1000.0, rate, completed / (wall_ms / 1000.0),
            bytes / (wall_ms * 1e-3) / 1e9, pct(0.50), pct(0.99), latencies.empty() ? 0.0 : latencies.back(), mean, (unsigned long long)verified,
-           (unsigned long long)mismatches, peak_used / double(1ull << 30), budget_gib, (unsigned long long)waits, streams, max_batch, host_admit,
-           host_step, host_poll, host_check);
+           (unsigned long long)mismatches, peak_used.load() / double(1ull << 30), budget_gib, (unsigned long long)waits.load(), streams, max_batch,
+           host_admit.load(), host_step, host_poll, host_check);
     ck(csx_server_destroy(server), "server destroy");
     ck(csx_arena_destroy(arena), "arena destroy");
     for (Check& c : checks) {
