@@ -25,7 +25,6 @@ COMMON = [
     "-O3",
     "-fPIC",
     "-Wall",
-    "-Wno-unused-function",
     f"-I{INCLUDE}",
     f"-I{CSRC}",
 ]
@@ -93,7 +92,7 @@ def _build_driver():
     for src, target, libs in (("csx_bench.cpp", DRIVER, ["-lcsx", "-lcsx_baseline"]), ("csx_serve.cpp", SERVE, ["-lcsx"]),
                               ("csx_hbm_probe.hip", HBM_PROBE, [])):
         tmp = target + ".tmp"
-        lang = ["-x", "hip", *DEVICE, "-Wno-unused-result"] if src.endswith(".hip") else [f"--offload-arch={ARCH}"]
+        lang = ["-x", "hip", *DEVICE] if src.endswith(".hip") else [f"--offload-arch={ARCH}"]
         cmd = [HIPCC, *lang, *COMMON, "-o", tmp, os.path.join(PKG, "driver", src), f"-L{PKG}", *libs, "-Wl,-rpath,$ORIGIN"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

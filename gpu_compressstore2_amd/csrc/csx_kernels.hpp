@@ -117,22 +117,6 @@ __global__ __launch_bounds__(kPopcBlock) void k_popc(const uint8_t* __restrict__
 }
 
 // ======================================================================
-// single-pass scans: shared init kernel
-// ======================================================================
-
-// Zero the ticket, the error word and `tiles` status words.  A plain kernel,
-// not a memset, so that streams captured into graphs replay it faithfully.
-static __global__ void k_lookback_init(WsHeader* hdr, uint64_t* status, uint64_t tiles)
-{
-    const uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (tid == 0) {
-        hdr->ticket = 0;
-        hdr->error = 0;
-    }
-    for (uint64_t i = tid; i < tiles; i += (uint64_t)gridDim.x * blockDim.x) status[i] = 0;
-}
-
-// ======================================================================
 // phase 2: exclusive scan of chunk counts (u32 -> u64), decoupled look-back
 // ======================================================================
 constexpr int kScanBlock = 256;
@@ -654,48 +638,6 @@ __global__ __launch_bounds__(256) void k_thread_per_chunk(TileArgs a)
                 }
             }
         }
-    }
-}
-
-// Write a 64-bit value from the device (graph-safe replacement for a memset
-// or a host copy of a single word).
-static __global__ void k_store_u64(uint64_t* dst, uint64_t v)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
-}
-
-// ======================================================================
-// the reference's up-sweep tree and its tree walk
-// ======================================================================
-
-// In place: pss[j] (a count on entry) becomes the sum of counts over
-// (j - lowbit(j+1), j], given the exclusive offsets `ex` of those counts.
-// Equals the state left by the log2(chunks) up-sweep launches of
-// src/kernels/kernel_3pass.cuh:69-153.
-static __global__ void k_tree_from_offsets(uint32_t* __restrict__ pss, const uint64_t* __restrict__ ex, uint64_t m)
-{
-    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < m; j += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t low = (j + 1) & (~(j + 1) + 1);
-        const uint64_t inclusive = ex[j] + pss[j];
-        pss[j] = (uint32_t)(inclusive - ex[j + 1 - low]);
-    }
-}
-
-// Exclusive offset of every chunk from the tree: the descent of
-// d_3pass_pproc_pssidx (src/kernels/kernel_3pass.cuh:155-169).
-static __global__ void k_tree_offsets(const uint32_t* __restrict__ tree, uint32_t* __restrict__ out, uint64_t m, uint64_t p2)
-{
-    for (uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; idx < m; idx += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t half = p2 / 2, consumed = 0;
-        uint32_t acc = 0;
-        while (half >= 1) {
-            if (idx >= consumed + half) {
-                acc += tree[consumed + half - 1];
-                consumed += half;
-            }
-            half /= 2;
-        }
-        out[idx] = acc;
     }
 }
 

@@ -73,11 +73,11 @@ template <typename F> float timeit(F f, hipEvent_t e0, hipEvent_t e1)
     f();
     std::vector<float> t;
     for (int r = 0; r < 9; r++) {
-        hipEventRecord(e0, 0);
+        (void)hipEventRecord(e0, 0);
         f();
-        hipEventRecord(e1, 0);
-        hipEventSynchronize(e1);
-        float ms; hipEventElapsedTime(&ms, e0, e1); t.push_back(ms);
+        (void)hipEventRecord(e1, 0);
+        (void)hipEventSynchronize(e1);
+        float ms; (void)hipEventElapsedTime(&ms, e0, e1); t.push_back(ms);
     }
     std::sort(t.begin(), t.end());
     return t[t.size() / 2];
@@ -112,8 +112,8 @@ int main()
     CK(hipMemset(in, 1, nvec * 16));
     CK(hipMemset(out, 0, nvec * 16));
     hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
     if (run<1, true>(in, out, sink, nvec, e0, e1)) return 1;
     if (run<4, true>(in, out, sink, nvec, e0, e1)) return 1;
     if (run<8, true>(in, out, sink, nvec, e0, e1)) return 1;
