@@ -525,7 +525,7 @@ static csx_status finish_host_count(uint64_t* h_count, const uint64_t* d_count_s
 {
     uint32_t err = 0;
     CSX_HIP(hipMemcpyAsync(h_count, d_count_src, sizeof(uint64_t), hipMemcpyDeviceToHost, stream), "count copy");
-    CSX_HIP(hipMemcpyAsync(&err, &hdr->error, sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "error copy");
+    if (hdr) CSX_HIP(hipMemcpyAsync(&err, &hdr->error, sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "error copy");
     CSX_HIP(hipStreamSynchronize(stream), "stream synchronize");
     if (err) return fail(CSX_ERR_DEVICE, "a look-back wait gave up (error word %u)", err);
     return ok();
@@ -569,6 +569,20 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
     if (n == 0) {
         CSX_HIP(csx::launch_lookback_init(hdr, status, 0, stream), "csx_compact init");
         CSX_HIP(csx::launch_store_u64(count_dst, 0, stream), "csx_compact count");
+    }
+    else if (n <= tile_elems(elem_size)) {
+        // the whole input is one tile: one workgroup, output offset 0, every pipeline alike
+        csx::TileArgs a{};
+        a.in = in;
+        a.mask = d_mask;
+        a.out = out;
+        a.n = n;
+        a.count_out = count_dst;
+        a.given_off = 0;
+        CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeGiven, true, bit_order, aligned, a, stream); }),
+                "csx_compact single tile");
+        if (h_count) return finish_host_count(h_count, count_dst, nullptr, stream);
+        return ok();
     }
     else if (pipeline == CSX_PIPELINE_FUSED) {
         CSX_HIP(csx::launch_lookback_init(hdr, status, p.fused_tiles, stream), "csx_compact init");

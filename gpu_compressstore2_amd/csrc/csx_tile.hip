@@ -24,6 +24,10 @@ static void launch_tile_impl(int mode, bool staged, int vpt, const TileArgs& a, 
     else if (mode == kModeFused) {
         k_tile<W, ORDER, ALIGNED, kModeFused, true><<<grid, 256, 0, s>>>(a);
     }
+    else if (mode == kModeGiven) {
+        // one tile at a.given_off (inputs of at most one tile: the whole call)
+        k_tile<W, ORDER, ALIGNED, kModeGiven, true><<<grid, 256, 0, s>>>(a);
+    }
     else if (staged) {
         // three-phase writeout: a tile's offset comes from the chunk offsets,
         // so any tile size works with the same workspace
