@@ -156,12 +156,8 @@ __device__ __forceinline__ uint64_t read_status(const uint64_t* status)
 
 // Called by all 64 lanes of ONE wave for tile `tile` > 0 whose aggregate has
 // already been published.  Returns the exclusive prefix of the tile (sum of
-// all earlier tiles).  Each round loads the status words of the K*64 closest
-// unresolved predecessors at once (lane i: tiles base - i - 64j, j < K), then
-// walks those windows nearest first, waiting only on windows it needs and
-// stopping at the first inclusive prefix; with many tiles in flight the
-// frontier of known prefixes then advances K*64 tiles per round trip.
-template <int K = 4>
+// all earlier tiles).  Each round inspects the 64 closest unresolved
+// predecessors; lane i looks at tile (base - i).
 __device__ __forceinline__ uint64_t lookback_exclusive(const uint64_t* status, uint64_t tile, uint32_t* err)
 {
     const uint32_t lane = lane_id();
@@ -169,32 +165,24 @@ __device__ __forceinline__ uint64_t lookback_exclusive(const uint64_t* status, u
     int64_t base = (int64_t)tile - 1;
     uint32_t spins = 0;
     while (true) {
-        uint64_t s[K];
-#pragma unroll
-        for (int j = 0; j < K; j++) {
-            const int64_t me = base - (int64_t)lane - (int64_t)kWave * j;
-            s[j] = me >= 0 ? read_status(status + me) : kFlagPrefix;  // before tile 0: prefix 0
-        }
-#pragma unroll
-        for (int j = 0; j < K; j++) {
-            const int64_t me = base - (int64_t)lane - (int64_t)kWave * j;
-            // wait until every predecessor of this window has at least its aggregate
-            while (__any((s[j] >> kFlagShift) == 0)) {
-                if (++spins > kSpinLimit) {
-                    if (lane == 0) atomicOr(err, 1u);
-                    return prefix;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                if ((s[j] >> kFlagShift) == 0) s[j] = read_status(status + me);
+        const int64_t me = base - (int64_t)lane;
+        uint64_t s = me >= 0 ? read_status(status + me) : kFlagPrefix;  // before tile 0: prefix 0
+        // wait until every inspected predecessor has at least its aggregate
+        while (__any((s >> kFlagShift) == 0)) {
+            if (++spins > kSpinLimit) {
+                if (lane == 0) atomicOr(err, 1u);
+                return prefix;
             }
-            const uint64_t is_p = __ballot((s[j] >> kFlagShift) == 2);
-            // closest predecessor carrying an inclusive prefix: lowest lane with P
-            const uint32_t stop = is_p ? (uint32_t)__builtin_ctzll(is_p) : (uint32_t)kWave;
-            const uint64_t contrib = lane <= stop ? (s[j] & kValueMask) : 0;
-            prefix += wave_sum(contrib);
-            if (is_p) return prefix;
+            __builtin_amdgcn_s_sleep(1);
+            if ((s >> kFlagShift) == 0) s = read_status(status + me);
         }
-        base -= (int64_t)kWave * K;
+        const uint64_t is_p = __ballot((s >> kFlagShift) == 2);
+        // closest predecessor carrying an inclusive prefix: lowest lane with P
+        const uint32_t stop = is_p ? (uint32_t)__builtin_ctzll(is_p) : (uint32_t)kWave;
+        const uint64_t contrib = lane <= stop ? (s & kValueMask) : 0;
+        prefix += wave_sum(contrib);
+        if (is_p) return prefix;
+        base -= kWave;
     }
 }
 
