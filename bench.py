@@ -205,7 +205,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the compaction shards "
-              "by contiguous ranges with one exchange of per-GPU counts; see DESIGN.md)", file=sys.stderr)
+              "by contiguous ranges with one exchange of per-GPU counts)", file=sys.stderr)
         sys.exit(2)
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
