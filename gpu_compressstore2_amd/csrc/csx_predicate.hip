@@ -8,8 +8,9 @@
 // so GT/GE/LT/LE/EQ/NE/BETWEEN share one kernel per element type, and IEEE
 // comparisons give NaN its usual answers (false, true for NE).
 //
-// Layout: one wave per chunk of 1024 elements (grid-stride over chunks).  The
-// chunk is W steps of 64 lanes x 16 bytes, all loads issued before any use.
+// Layout: a wave takes two chunks of 1024 elements per step (grid-stride over
+// chunk pairs).  A chunk is W rows of 64 lanes x 16 bytes; both chunks' loads
+// (non-temporal: the column is read once) are issued before any use.
 // A lane's 16/W elements give 16/W bits; OR-shuffles across the 2W lanes of a
 // 32-element word assemble the word, and lane j < 32 collects the chunk's word
 // j, so the chunk's 128 mask bytes leave in one coalesced store.  The chunk's
@@ -22,6 +23,7 @@
 namespace csx {
 
 constexpr uint32_t kPredChunk = 1024;
+constexpr uint32_t kPredPair = 2;  // chunks per wave step
 constexpr int kPredBlock = 256;
 
 template <typename T> __device__ __forceinline__ T from_bits(uint64_t b)
@@ -67,42 +69,49 @@ __global__ __launch_bounds__(kPredBlock) void k_predicate(const T* __restrict__ 
     const uint64_t waves = (uint64_t)gridDim.x * (kPredBlock / kWave);
     uint64_t selected = 0;
 
-    for (uint64_t c = wave0; c < chunks; c += waves) {
-        const uint64_t cbase = c * kPredChunk;
+    for (uint64_t c0 = wave0 * kPredPair; c0 < chunks; c0 += waves * kPredPair) {
         union {
             T e[V];
             u32x4 v;
-        } x[S];
+        } x[kPredPair][S];
 #pragma unroll
-        for (uint32_t s = 0; s < S; s++) {
-            const uint64_t e = cbase + s * (64 * V) + lane * V;
-            if (ALIGNED && e + V <= n) {
-                x[s].v = *reinterpret_cast<const u32x4*>(in + e);
-            }
-            else {
+        for (uint32_t q = 0; q < kPredPair; q++) {
 #pragma unroll
-                for (uint32_t j = 0; j < V; j++) x[s].e[j] = e + j < n ? in[e + j] : T(0);
+            for (uint32_t s = 0; s < S; s++) {
+                const uint64_t e = (c0 + q) * kPredChunk + s * (64 * V) + lane * V;
+                if (ALIGNED && e + V <= n) {
+                    x[q][s].v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(in + e));
+                }
+                else {
+#pragma unroll
+                    for (uint32_t j = 0; j < V; j++) x[q][s].e[j] = e + j < n ? in[e + j] : T(0);
+                }
             }
         }
-        uint32_t word = 0;
 #pragma unroll
-        for (uint32_t s = 0; s < S; s++) {
-            const uint64_t e = cbase + s * (64 * V) + lane * V;
-            uint32_t bits = 0;
+        for (uint32_t q = 0; q < kPredPair; q++) {
+            const uint64_t c = c0 + q;
+            if (c >= chunks) break;
+            uint32_t word = 0;
 #pragma unroll
-            for (uint32_t j = 0; j < V; j++) bits |= (uint32_t)(e + j < n && pred_eval<T>(x[s].e[j], lo, hi, flags)) << j;
-            // word s*WPS + lane/L of the chunk: lanes of one group hold disjoint bit ranges
-            uint32_t w = bits << ((lane % L) * V);
+            for (uint32_t s = 0; s < S; s++) {
+                const uint64_t e = c * kPredChunk + s * (64 * V) + lane * V;
+                uint32_t bits = 0;
 #pragma unroll
-            for (uint32_t o = 1; o < L; o <<= 1) w |= __shfl_xor(w, o, kWave);
-            // lane j < 32 takes chunk word j when step s makes it (lanes >= 32 never match)
-            const uint32_t got = __shfl(w, (lane % WPS) * L, kWave);
-            if (lane / WPS == s) word = got;
+                for (uint32_t j = 0; j < V; j++) bits |= (uint32_t)(e + j < n && pred_eval<T>(x[q][s].e[j], lo, hi, flags)) << j;
+                // word s*WPS + lane/L of the chunk: lanes of one group hold disjoint bit ranges
+                uint32_t w = bits << ((lane % L) * V);
+#pragma unroll
+                for (uint32_t o = 1; o < L; o <<= 1) w |= __shfl_xor(w, o, kWave);
+                // lane j < 32 takes chunk word j when step s makes it (lanes >= 32 never match)
+                const uint32_t got = __shfl(w, (lane % WPS) * L, kWave);
+                if (lane / WPS == s) word = got;
+            }
+            if (lane < 32) store_mask_word<ORDER>(mask, c * 32 + lane, word, nbytes);  // bits at index >= n are 0
+            const uint32_t cnt = wave_sum<uint32_t>(lane < 32 ? popc32(word) : 0u);
+            if (counts && lane == 0) counts[c] = cnt;
+            selected += cnt;
         }
-        if (lane < 32) store_mask_word<ORDER>(mask, c * 32 + lane, word, nbytes);  // bits at index >= n are 0
-        const uint32_t cnt = wave_sum<uint32_t>(lane < 32 ? popc32(word) : 0u);
-        if (counts && lane == 0) counts[c] = cnt;
-        selected += cnt;
     }
     if (total) {
         __shared__ unsigned long long acc;
@@ -119,13 +128,14 @@ static hipError_t launch_pred_t(const void* in, uint64_t n, const PredSpec& p, u
                                 uint64_t* status, uint64_t init_tiles, hipStream_t s)
 {
     const uint64_t chunks = (n + kPredChunk - 1) / kPredChunk;
-    uint64_t g = (chunks + kPredBlock / kWave - 1) / (kPredBlock / kWave);
+    const uint64_t per_block = kPredPair * (kPredBlock / kWave);
+    uint64_t g = (chunks + per_block - 1) / per_block;
     if (g < 1) g = 1;
-    // one chunk per wave up to 2^28 elements (one-pass grids stream fastest on
+    // one step per wave up to 2^28 elements (one-pass grids stream fastest on
     // this HBM); 1-byte columns are ALU-bound and do best with 4 chunks per wave.
     // A running total costs one same-address atomic per block, which the L2
     // serialises (~16 ns each), so counting grids stay at 4096 blocks.
-    const uint64_t cap = total ? 4096 : sizeof(T) == 1 ? 16384 : 65536;
+    const uint64_t cap = total ? 4096 : sizeof(T) == 1 ? 8192 : 32768;
     if (g > cap) g = cap;
     const bool aligned = ((uintptr_t)in & 15) == 0;
     auto x = static_cast<const T*>(in);
