@@ -69,3 +69,30 @@ def test_full_size(O, w, kind, view):
         assert bool(((expect.to(torch.int64) & 0xFFFFFFFF) > 2 ** 31).all())
         _, k = csx.compact_predicate(flat, ">", 2 ** 31, n=N, col_type=2, out=out)
         assert k == expect.numel() and torch.equal(out[:k], expect)
+
+
+def test_beyond_32_bit_index():
+    """A u8 column longer than 2^32 elements (a ragged tail byte included):
+    offsets, counts and tile indices are 64-bit on every pipeline."""
+    n = (1 << 32) + 4099
+    data = csx.gen_values(n, 1, csx.VALUES_INTEGER, seed=77)
+    mask = csx.gen_mask_bernoulli(n, 0.5, seed=78)
+    sel = unpack(mask, n)
+    # torch.masked_select overflows past 2^32 elements: select slice by slice
+    step = 1 << 30
+    expect = torch.cat([torch.masked_select(data[a:a + step], sel[a:a + step]) for a in range(0, n, step)])
+    del sel
+    k_exp = expect.numel()
+    assert k_exp > (1 << 31)
+    out = torch.empty_like(data)
+    ws = csx.alloc_workspace(csx.auto_workspace_size(n, 1))
+    for pipeline in (csx.PIPELINE_THREE_PHASE, csx.PIPELINE_TWO_PASS):
+        print(f"[beyond-2^32] n={n} pipeline={pipeline} k={k_exp}", flush=True)
+        out[k_exp:k_exp + 4096].fill_(0xA5)  # nothing past the count is written
+        _, k = csx.compact(data, mask, n=n, pipeline=pipeline, out=out, workspace=ws)
+        assert k == k_exp, pipeline
+        assert torch.equal(out[:k], expect), pipeline
+        assert bool((out[k:k + 4096] == 0xA5).all()), pipeline
+    out.fill_(0)
+    _, k = csx.compact_auto(data, mask, 0.5, n=n, out=out, workspace=ws)
+    assert k == k_exp and torch.equal(out[:k], expect)
