@@ -196,11 +196,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-column batched measurement")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline")
     ap.add_argument("--sels", default="0.01,0.5,0.99", help="selectivities to sweep (0.5 is always included)")
-    ap.add_argument("--pipelines", default="three_phase,two_pass,fused", help="pipelines to sweep (three_phase is always included)")
+    ap.add_argument("--pipelines", default="three_phase,two_pass,fused,sparse", help="pipelines to sweep (three_phase is always included)")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the serving-loop run (0 skips it)")
     args = ap.parse_args()
     sels = sorted(set(float(x) for x in args.sels.split(",")) | {0.5})
-    all_pipes = {"three_phase": csx.PIPELINE_THREE_PHASE, "two_pass": csx.PIPELINE_TWO_PASS, "fused": csx.PIPELINE_FUSED}
+    all_pipes = {"three_phase": csx.PIPELINE_THREE_PHASE, "two_pass": csx.PIPELINE_TWO_PASS, "fused": csx.PIPELINE_FUSED, "sparse": csx.PIPELINE_SPARSE}
     pipes = [(k, all_pipes[k]) for k in all_pipes if k == "three_phase" or k in args.pipelines.split(",")]
 
     if args.gpus != 1:

@@ -30,10 +30,12 @@ LSB_FIRST = 1
 WRITEOUT_STAGED = 0
 WRITEOUT_DIRECT = 1
 WRITEOUT_THREAD_PER_CHUNK = 2
+WRITEOUT_SPARSE = 3
 
 PIPELINE_FUSED = 0
 PIPELINE_THREE_PHASE = 1
 PIPELINE_TWO_PASS = 2
+PIPELINE_SPARSE = 3
 
 MAX_ELEMENTS = 1 << 40
 

@@ -189,6 +189,16 @@ hipError_t dispatch_tpc(uint32_t w, int order, const csx::TileArgs& a, hipStream
     }
 }
 
+hipError_t dispatch_sparse(uint32_t w, int order, bool mask_aligned, const csx::TileArgs& a, hipStream_t s)
+{
+    switch (w) {
+        case 1: return csx::launch_sparse<1>(order, mask_aligned, a, s);
+        case 2: return csx::launch_sparse<2>(order, mask_aligned, a, s);
+        case 4: return csx::launch_sparse<4>(order, mask_aligned, a, s);
+        default: return csx::launch_sparse<8>(order, mask_aligned, a, s);
+    }
+}
+
 bool launch_fits(uint64_t tiles) { return tiles * 256ull < (1ull << 32); }
 
 // ---- dominant-kernel timing (csx_profile_begin / csx_profile_end) ----
@@ -453,7 +463,7 @@ csx_status csx_writeout_ex(const csx_writeout_args* wa, hipStream_t stream)
     const int variant = wa->variant, bit_order = wa->bit_order;
     csx_status s;
     if ((s = check_width(elem_size)) || (s = check_chunk(chunk_len)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
-    if (variant < CSX_WRITEOUT_STAGED || variant > CSX_WRITEOUT_THREAD_PER_CHUNK) return fail(CSX_ERR_INVALID, "unknown writeout variant %d", variant);
+    if (variant < CSX_WRITEOUT_STAGED || variant > CSX_WRITEOUT_SPARSE) return fail(CSX_ERR_INVALID, "unknown writeout variant %d", variant);
     if (n == 0) return ok();
     if (!wa->in || !wa->mask || !wa->offsets || !wa->out) return fail(CSX_ERR_INVALID, "null input, mask, offsets or output pointer");
     if (((uintptr_t)wa->in % elem_size) || ((uintptr_t)wa->out % elem_size))
@@ -472,6 +482,11 @@ csx_status csx_writeout_ex(const csx_writeout_args* wa, hipStream_t stream)
     a.out_base = wa->out_base;
     if (variant == CSX_WRITEOUT_THREAD_PER_CHUNK) {
         CSX_HIP(timed(stream, [&] { return dispatch_tpc(elem_size, bit_order, a, stream); }), "csx_writeout launch");
+        return ok();
+    }
+    if (variant == CSX_WRITEOUT_SPARSE) {
+        const bool mask_aligned = ((uintptr_t)wa->mask & 15) == 0;
+        CSX_HIP(timed(stream, [&] { return dispatch_sparse(elem_size, bit_order, mask_aligned, a, stream); }), "csx_writeout launch");
         return ok();
     }
     const uint64_t tiles = (n + tile_elems(elem_size) - 1) / tile_elems(elem_size);
@@ -544,7 +559,7 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
 {
     csx_status s;
     if ((s = check_width(elem_size)) || (s = check_chunk(chunk_len)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
-    if (pipeline != CSX_PIPELINE_FUSED && pipeline != CSX_PIPELINE_THREE_PHASE && pipeline != CSX_PIPELINE_TWO_PASS)
+    if (pipeline != CSX_PIPELINE_FUSED && pipeline != CSX_PIPELINE_THREE_PHASE && pipeline != CSX_PIPELINE_TWO_PASS && pipeline != CSX_PIPELINE_SPARSE)
         return fail(CSX_ERR_INVALID, "unknown pipeline %d", pipeline);
     if ((s = check_vpt(vpt, elem_size))) return s;
     if (pipeline != CSX_PIPELINE_THREE_PHASE && vpt != 8) return fail(CSX_ERR_INVALID, "only the three-phase writeout takes a tile size other than 8");
@@ -627,8 +642,14 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
         a.offsets = offsets;
         a.counts = counts;
         a.log2c = log2c;
-        CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeWriteout, true, bit_order, aligned, a, stream, (int)vpt); }),
-                "csx_compact writeout");
+        if (pipeline == CSX_PIPELINE_SPARSE) {
+            const bool mask_aligned = ((uintptr_t)d_mask & 15) == 0;
+            CSX_HIP(timed(stream, [&] { return dispatch_sparse(elem_size, bit_order, mask_aligned, a, stream); }), "csx_compact sparse writeout");
+        }
+        else {
+            CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeWriteout, true, bit_order, aligned, a, stream, (int)vpt); }),
+                    "csx_compact writeout");
+        }
     }
     if (h_count) return finish_host_count(h_count, count_dst, hdr, stream);
     return ok();
@@ -1234,6 +1255,8 @@ csx_status csx_autotune(uint32_t elem_size, uint64_t n, double selectivity, uint
         for (uint32_t vpt : {2u, 4u, 8u, 16u})
             if (vpt != 16 || elem_size >= 2) cand.push_back(csx_config{CSX_PIPELINE_THREE_PHASE, chunk, vpt, 0});
     cand.push_back(csx_config{CSX_PIPELINE_TWO_PASS, 1024, 8, 0});
+    cand.push_back(csx_config{CSX_PIPELINE_SPARSE, 1024, 8, 0});
+    cand.push_back(csx_config{CSX_PIPELINE_SPARSE, 4096, 8, 0});
     cand.push_back(csx_config{CSX_PIPELINE_FUSED, 1024, 8, 0});
 
     double best_t = 0.0;
@@ -1328,7 +1351,8 @@ csx_status csx_tune_load(const char* path)
         uint32_t w, vpt, cb;
         if (sscanf(line, "compact %u %u %lf %d %u %u %lf", &c.w, &c.log2n, &c.sel, &c.cfg.pipeline, &c.cfg.chunk_len, &c.cfg.vpt, &c.ms) == 7) {
             if (check_width(c.w) || check_chunk(c.cfg.chunk_len) || check_vpt(c.cfg.vpt, c.w) || c.log2n > 40 || !(c.sel >= 0 && c.sel <= 1) ||
-                (c.cfg.pipeline != CSX_PIPELINE_THREE_PHASE && c.cfg.pipeline != CSX_PIPELINE_TWO_PASS && c.cfg.pipeline != CSX_PIPELINE_FUSED) ||
+                (c.cfg.pipeline != CSX_PIPELINE_THREE_PHASE && c.cfg.pipeline != CSX_PIPELINE_TWO_PASS && c.cfg.pipeline != CSX_PIPELINE_FUSED &&
+                 c.cfg.pipeline != CSX_PIPELINE_SPARSE) ||
                 (c.cfg.pipeline != CSX_PIPELINE_THREE_PHASE && c.cfg.vpt != 8)) {
                 st = fail(CSX_ERR_INVALID, "%s:%u: configuration out of range", path, lineno);
                 break;

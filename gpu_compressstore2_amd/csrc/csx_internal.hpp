@@ -63,6 +63,8 @@ template <int W>
 hipError_t launch_tile_counts(int order, bool aligned, const uint8_t* mask, uint64_t n, uint32_t* tcounts, uint64_t* gsums, uint32_t log2g,
                               WsHeader* hdr, hipStream_t s);
 template <int W> hipError_t launch_thread_per_chunk(int order, const TileArgs& a, hipStream_t s);
+// sparse writeout (three-phase offsets, no LDS); mask_aligned: 16-byte mask pointer
+template <int W> hipError_t launch_sparse(int order, bool mask_aligned, const TileArgs& a, hipStream_t s);
 
 extern template hipError_t launch_tile<1>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
 extern template hipError_t launch_tile<2>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
@@ -70,11 +72,15 @@ extern template hipError_t launch_tile<4>(int, bool, int, bool, const TileArgs&,
 extern template hipError_t launch_tile<8>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
 extern template hipError_t launch_tile_counts<1>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, WsHeader*, hipStream_t);
 extern template hipError_t launch_thread_per_chunk<1>(int, const TileArgs&, hipStream_t);
+extern template hipError_t launch_sparse<1>(int, bool, const TileArgs&, hipStream_t);
 extern template hipError_t launch_tile_counts<2>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, WsHeader*, hipStream_t);
 extern template hipError_t launch_thread_per_chunk<2>(int, const TileArgs&, hipStream_t);
+extern template hipError_t launch_sparse<2>(int, bool, const TileArgs&, hipStream_t);
 extern template hipError_t launch_tile_counts<4>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, WsHeader*, hipStream_t);
 extern template hipError_t launch_thread_per_chunk<4>(int, const TileArgs&, hipStream_t);
+extern template hipError_t launch_sparse<4>(int, bool, const TileArgs&, hipStream_t);
 extern template hipError_t launch_tile_counts<8>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, WsHeader*, hipStream_t);
 extern template hipError_t launch_thread_per_chunk<8>(int, const TileArgs&, hipStream_t);
+extern template hipError_t launch_sparse<8>(int, bool, const TileArgs&, hipStream_t);
 
 } // namespace csx

@@ -642,6 +642,97 @@ __global__ __launch_bounds__(256) void k_thread_per_chunk(TileArgs a)
 }
 
 // ======================================================================
+// phase 3, sparse variant: no LDS, for low selectivity
+// ======================================================================
+//
+// A wave covers a step of 64 x 128 elements: lane l owns the 16-byte mask
+// vector of elements [128 l, 128 l + 128) of the step.  A wave scan of the
+// lanes' popcounts gives each lane its first output rank; the lane then
+// gathers its selected elements (up to four loads in flight) and stores them
+// one by one.  Without the staging buffer a workgroup holds no LDS, so many
+// more steps are in flight than tiles of the staged writeout, whose LDS is
+// sized for a fully selected tile: below a few percent selectivity, where the
+// staged tile waits on load latency, this form is faster; above, the
+// per-element gathers lose to whole-vector loads.
+constexpr uint64_t kSparseStep = 64 * 128;
+
+template <int W, int ORDER, bool ALIGNED>
+__global__ __launch_bounds__(256) void k_sparse(TileArgs a)
+{
+    const uint8_t* __restrict__ in = a.in;
+    uint8_t* __restrict__ out = a.out;
+    const uint64_t n = a.n;
+    const uint64_t nbytes = (n + 7) / 8;
+    const uint32_t lane = lane_id();
+    const uint64_t steps = (n + kSparseStep - 1) / kSparseStep;
+    const uint64_t base_out = a.out_base ? *a.out_base : 0ull;
+    const uint64_t waves = (uint64_t)gridDim.x * (256 / kWave);
+    for (uint64_t step = (blockIdx.x * 256ull + threadIdx.x) / kWave; step < steps; step += waves) {
+        const uint64_t e0 = step * kSparseStep;
+        const uint64_t el = e0 + 128ull * lane;
+        uint32_t w[4] = {0, 0, 0, 0};
+        if (el < n) {
+            const u32x4 x = load_mask16<ALIGNED>(a.mask, el / 8, nbytes);
+            w[0] = to_lsb_word<ORDER>(x.x);
+            w[1] = to_lsb_word<ORDER>(x.y);
+            w[2] = to_lsb_word<ORDER>(x.z);
+            w[3] = to_lsb_word<ORDER>(x.w);
+            if (el + 128 > n) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint64_t b = el + 32 * k;
+                    w[k] = b >= n ? 0u : low_bits(w[k], (uint32_t)(n - b));
+                }
+            }
+        }
+        // output offset of the step: its chunk's offset, plus the selected
+        // elements of the chunk before the step when the chunk is longer
+        const uint64_t chunk = e0 >> a.log2c;
+        const uint64_t cstart = chunk << a.log2c;
+        uint64_t before = 0;
+        if (cstart < e0) {
+            // whole 16-byte vectors below e0 < n: no tail bits among them
+            uint32_t c = 0;
+            for (uint64_t v = cstart / 128 + lane; v < e0 / 128; v += kWave) {
+                const u32x4 x = load_mask16<ALIGNED>(a.mask, v * 16, nbytes);
+                c += popc32(x.x) + popc32(x.y) + popc32(x.z) + popc32(x.w);
+            }
+            before = wave_sum((uint64_t)c);
+        }
+        const uint32_t cnt = popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
+        uint64_t r = base_out + (a.offsets ? a.offsets[chunk] : (uint64_t)a.offsets32[chunk]) + before + wave_inclusive_sum(cnt) - cnt;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint32_t bits = w[k];
+            const uint8_t* src = in + (el + 32 * k) * W;
+            while (bits) {
+                // up to four selected elements per round, their loads issued together
+                const uint32_t q0 = __builtin_ctz(bits);
+                bits &= bits - 1;
+                const bool h1 = bits != 0;
+                const uint32_t q1 = h1 ? __builtin_ctz(bits) : 0u;
+                bits &= h1 ? bits - 1 : bits;
+                const bool h2 = bits != 0;
+                const uint32_t q2 = h2 ? __builtin_ctz(bits) : 0u;
+                bits &= h2 ? bits - 1 : bits;
+                const bool h3 = bits != 0;
+                const uint32_t q3 = h3 ? __builtin_ctz(bits) : 0u;
+                bits &= h3 ? bits - 1 : bits;
+                const uint64_t v0 = load_elem<W>(src + q0 * W);
+                const uint64_t v1 = h1 ? load_elem<W>(src + q1 * W) : 0ull;
+                const uint64_t v2 = h2 ? load_elem<W>(src + q2 * W) : 0ull;
+                const uint64_t v3 = h3 ? load_elem<W>(src + q3 * W) : 0ull;
+                store_elem<W>(out + r * W, v0);
+                if (h1) store_elem<W>(out + (r + 1) * W, v1);
+                if (h2) store_elem<W>(out + (r + 2) * W, v2);
+                if (h3) store_elem<W>(out + (r + 3) * W, v3);
+                r += 1u + h1 + h2 + h3;
+            }
+        }
+    }
+}
+
+// ======================================================================
 // bytewise comparison (src/kernels/data_generator.cuh:122-161)
 // ======================================================================
 template <int W>

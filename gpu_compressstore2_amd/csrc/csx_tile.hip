@@ -101,7 +101,29 @@ template <int W> hipError_t launch_thread_per_chunk(int order, const TileArgs& a
     return hipGetLastError();
 }
 
+template <int W> hipError_t launch_sparse(int order, bool mask_aligned, const TileArgs& a, hipStream_t s)
+{
+    const uint64_t steps = (a.n + kSparseStep - 1) / kSparseStep;
+    uint64_t g = (steps + 3) / 4;  // one step per wave
+    if (g < 1) g = 1;
+    if (g > (1u << 22)) g = 1u << 22;  // beyond 2^37 elements the waves loop
+    if (order == kBitMsb) {
+        if (mask_aligned)
+            k_sparse<W, kBitMsb, true><<<(unsigned)g, 256, 0, s>>>(a);
+        else
+            k_sparse<W, kBitMsb, false><<<(unsigned)g, 256, 0, s>>>(a);
+    }
+    else {
+        if (mask_aligned)
+            k_sparse<W, kBitLsb, true><<<(unsigned)g, 256, 0, s>>>(a);
+        else
+            k_sparse<W, kBitLsb, false><<<(unsigned)g, 256, 0, s>>>(a);
+    }
+    return hipGetLastError();
+}
+
 template hipError_t launch_tile<CSX_TILE_W>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
+template hipError_t launch_sparse<CSX_TILE_W>(int, bool, const TileArgs&, hipStream_t);
 template hipError_t launch_tile_counts<CSX_TILE_W>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, WsHeader*, hipStream_t);
 template hipError_t launch_thread_per_chunk<CSX_TILE_W>(int, const TileArgs&, hipStream_t);
 

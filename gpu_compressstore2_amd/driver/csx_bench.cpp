@@ -276,6 +276,7 @@ template <typename T> void run(Opts o)
                         [&](int cs, int, int) { return bench_csx_pipeline(&id, CSX_PIPELINE_THREE_PHASE, d_input, d_mask, d_output, n, cs); });
     benchs.emplace_back("csx_two_pass", [&](int cs, int, int) { return bench_csx_pipeline(&id, CSX_PIPELINE_TWO_PASS, d_input, d_mask, d_output, n, cs); });
     benchs.emplace_back("csx_fused", [&](int cs, int, int) { return bench_csx_pipeline(&id, CSX_PIPELINE_FUSED, d_input, d_mask, d_output, n, cs); });
+    benchs.emplace_back("csx_sparse", [&](int cs, int, int) { return bench_csx_pipeline(&id, CSX_PIPELINE_SPARSE, d_input, d_mask, d_output, n, cs); });
     if (!o.only.empty()) {
         std::vector<std::pair<std::string, Fn>> keep;
         std::stringstream ss(o.only);

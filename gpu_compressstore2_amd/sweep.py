@@ -20,8 +20,8 @@ import torch
 
 import gpu_compressstore2_amd as csx
 
-VARIANTS = {"staged": csx.WRITEOUT_STAGED, "direct": csx.WRITEOUT_DIRECT, "thread_per_chunk": csx.WRITEOUT_THREAD_PER_CHUNK}
-PIPELINES = {"three_phase": csx.PIPELINE_THREE_PHASE, "two_pass": csx.PIPELINE_TWO_PASS, "fused": csx.PIPELINE_FUSED}
+VARIANTS = {"staged": csx.WRITEOUT_STAGED, "direct": csx.WRITEOUT_DIRECT, "thread_per_chunk": csx.WRITEOUT_THREAD_PER_CHUNK, "sparse": csx.WRITEOUT_SPARSE}
+PIPELINES = {"three_phase": csx.PIPELINE_THREE_PHASE, "two_pass": csx.PIPELINE_TWO_PASS, "fused": csx.PIPELINE_FUSED, "sparse": csx.PIPELINE_SPARSE}
 
 
 def median_ms(fn, reps):

@@ -62,8 +62,11 @@ typedef enum {
                                           (replaces the optimized writeout, kernel_3pass.cuh:329-467) */
     CSX_WRITEOUT_DIRECT = 1,           /* same ranks, each element stored straight from registers
                                           (replaces the naive writeout, kernel_3pass.cuh:216-327) */
-    CSX_WRITEOUT_THREAD_PER_CHUNK = 2  /* one thread walks one chunk bit by bit
+    CSX_WRITEOUT_THREAD_PER_CHUNK = 2, /* one thread walks one chunk bit by bit
                                           (replaces proc_none, kernel_3pass.cuh:578-639) */
+    CSX_WRITEOUT_SPARSE = 3            /* no LDS: a lane per 128 elements gathers its selected elements one by one,
+                                          ranks from a wave scan; the fastest below a few percent selectivity.
+                                          No counterpart in the reference. */
 } csx_writeout_variant;
 
 /* whole-pipeline forms of csx_compact */
@@ -71,8 +74,9 @@ typedef enum {
     CSX_PIPELINE_FUSED = 0,       /* one pass: mask popcount, decoupled look-back and staged writeout per tile */
     CSX_PIPELINE_THREE_PHASE = 1, /* csx_popc -> csx_scan -> csx_writeout (STAGED), the reference's bench10 shape;
                                      the fastest form at low and medium selectivity (default of the bindings) */
-    CSX_PIPELINE_TWO_PASS = 2     /* pass 1 counts every tile and group of tiles from the mask; pass 2 is the staged
+    CSX_PIPELINE_TWO_PASS = 2,    /* pass 1 counts every tile and group of tiles from the mask; pass 2 is the staged
                                      writeout, each tile reducing its own offset from those counts */
+    CSX_PIPELINE_SPARSE = 3       /* csx_popc -> csx_scan -> csx_writeout (SPARSE): for low selectivity */
 } csx_pipeline;
 
 /* Largest element count accepted (offsets are 62-bit in the look-back status words). */
@@ -162,7 +166,7 @@ csx_status csx_writeout_w8(const uint64_t* d_in, const uint8_t* d_mask, const ui
  * k written to *d_count (device, nullable) and, if h_count is not null, to
  * *h_count after synchronising the stream.  d_out must hold k elements
  * (n is always enough); nothing at or past d_out[k] is written.
- * chunk_len only matters for CSX_PIPELINE_THREE_PHASE.
+ * chunk_len only matters for CSX_PIPELINE_THREE_PHASE and CSX_PIPELINE_SPARSE.
  * CSX_PIPELINE_THREE_PHASE is the default of the language bindings.
  * Replaces the benchN compositions (src/benchmarks.cuh:153-440) -- bench10
  * is CSX_PIPELINE_THREE_PHASE with the staged writeout. */
@@ -272,8 +276,8 @@ uint32_t csx_server_inflight(const csx_server* server);
 /* ---------------------------------------------------------------- tuning
  * A compaction configuration.  pipeline: CSX_PIPELINE_*; chunk_len: the
  * three-phase chunk length; vpt: tile size in 16-byte vectors per thread of
- * the three-phase writeout (2, 4, 8, or 16 when elem_size >= 2; TWO_PASS and
- * FUSED always use 8) or of the batched form (2, 4 or 8); count_blocks: grid
+ * the three-phase writeout (2, 4, 8, or 16 when elem_size >= 2; TWO_PASS,
+ * FUSED and SPARSE take 8, which they do not use) or of the batched form (2, 4 or 8); count_blocks: grid
  * bound of the batched form's per-tile count kernel (0: 4096).
  * The tuning table maps (elem_size, log2 n, selectivity) to the fastest
  * measured configuration; csx_tune_lookup answers with the measured cell of

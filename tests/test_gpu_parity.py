@@ -82,7 +82,7 @@ def resolve_n(n, w):
     return {"T-1": T - 1, "T": T, "T+1": T + 1, "3T+17": 3 * T + 17}[n]
 
 
-PIPELINES = [csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE, csx.PIPELINE_TWO_PASS]
+PIPELINES = [csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE, csx.PIPELINE_TWO_PASS, csx.PIPELINE_SPARSE]
 
 
 @pytest.mark.parametrize("pipeline", PIPELINES)
@@ -171,7 +171,7 @@ def test_scan_tree_and_walk(O, m):
     assert (offs == csum).all()
 
 
-@pytest.mark.parametrize("variant", [csx.WRITEOUT_STAGED, csx.WRITEOUT_DIRECT, csx.WRITEOUT_THREAD_PER_CHUNK])
+@pytest.mark.parametrize("variant", [csx.WRITEOUT_STAGED, csx.WRITEOUT_DIRECT, csx.WRITEOUT_THREAD_PER_CHUNK, csx.WRITEOUT_SPARSE])
 @pytest.mark.parametrize("w", [1, 2, 4, 8])
 def test_writeout_variants(O, w, variant):
     for chunk, n, kind, skip in ((32, 70001, "bern50", False), (1024, 250007, "zipf", True), (4096, 123457, "runs", True),
@@ -305,7 +305,7 @@ def test_scan_u32_in_place_and_separate(O):
         assert int(tot2.item()) == total and (host(d, np.uint32) == expect.astype(np.uint32)).all()
 
 
-@pytest.mark.parametrize("variant", [csx.WRITEOUT_STAGED, csx.WRITEOUT_DIRECT, csx.WRITEOUT_THREAD_PER_CHUNK])
+@pytest.mark.parametrize("variant", [csx.WRITEOUT_STAGED, csx.WRITEOUT_DIRECT, csx.WRITEOUT_THREAD_PER_CHUNK, csx.WRITEOUT_SPARSE])
 def test_writeout_u32_offsets_and_out_base(O, variant):
     n, chunk = 200003, 512
     for w in (1, 4, 8):

@@ -20,6 +20,8 @@ def candidates(w):
                 out.append(csx.Config(csx.PIPELINE_THREE_PHASE, chunk, vpt, 0))
     out.append(csx.Config(csx.PIPELINE_TWO_PASS, 1024, 8, 0))
     out.append(csx.Config(csx.PIPELINE_FUSED, 1024, 8, 0))
+    out.append(csx.Config(csx.PIPELINE_SPARSE, 1024, 8, 0))
+    out.append(csx.Config(csx.PIPELINE_SPARSE, 4096, 8, 0))
     return out
 
 
