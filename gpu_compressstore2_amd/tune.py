@@ -3,7 +3,8 @@
     python -m gpu_compressstore2_amd.tune [--out FILE] [--reps R]
 
 For every element width (1, 2, 4, 8 bytes), size (2^16, 2^20, 2^22, 2^24,
-2^26, 2^28 elements) and Bernoulli selectivity (1, 10, 25, 50, 75, 90, 99 %)
+2^26, 2^28 elements) and Bernoulli selectivity (0.1, 1, 3, 10, 25, 50, 75, 90,
+99 %)
 csx_autotune times
 every candidate configuration on device-generated inputs and records the
 fastest; the table is written as text (csx_tune_save).  The package loads
@@ -19,7 +20,7 @@ import gpu_compressstore2_amd as csx
 
 WIDTHS = (1, 2, 4, 8)
 LOG2N = (16, 20, 22, 24, 26, 28)
-SELS = (0.01, 0.1, 0.25, 0.5, 0.75, 0.9, 0.99)
+SELS = (0.001, 0.01, 0.03, 0.1, 0.25, 0.5, 0.75, 0.9, 0.99)
 PIPELINES = {0: "fused", 1: "three_phase", 2: "two_pass", 3: "sparse"}
 
 
