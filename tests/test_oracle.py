@@ -219,3 +219,21 @@ def test_pattern_from_selectivity(O):
         p = O.pattern_from_selectivity(L, s)
         assert bin(p).count("1") == int(L * np.float32(s))
         assert p & ((1 << (32 - L)) - 1) == 0
+
+
+def test_pattern_from_selectivity_from_reference_stream(O, golden):
+    """src/main.cu:54-64 draws r from fast_prng(42) and sets bit 31 - (r mod L)
+    unless it is set, until int(L * s) bits are set: the set positions are the
+    first int(L * s) distinct values of r mod L in the stream.  The stream is
+    the reference's own (the seed-42 golden vector written from oracle/_ref),
+    so this pins the loop, its modulus, bit placement and count rounding
+    independently of the oracle's PRNG."""
+    stream = next(unhex(c["out"], np.uint32) for c in golden if c["fn"] == "prng_seq" and c["seed"] == 42)
+    for L, s in ((8, 0.5), (32, 0.25), (32, 0.5), (7, 0.99), (16, 0.75), (5, 0.2), (3, 0.7), (1, 1.0), (12, 0.0)):
+        ones = int(np.float32(L) * np.float32(s))  # the reference's int * float, truncated
+        distinct = list(dict.fromkeys(int(r) % L for r in stream))[:ones]
+        assert len(distinct) == ones  # 64 reference draws cover these cases
+        expect = 0
+        for i in distinct:
+            expect |= 1 << (31 - i)
+        assert O.pattern_from_selectivity(L, s) == expect, (L, s)
