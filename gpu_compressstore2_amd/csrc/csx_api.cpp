@@ -724,8 +724,31 @@ csx_status csx_pattern_compact(const void* d_in, void* d_out, uint64_t n, uint32
     if (k > 0 && (!d_in || !d_out)) return fail(CSX_ERR_INVALID, "null input or output pointer");
     if (((uintptr_t)d_in % elem_size) || ((uintptr_t)d_out % elem_size))
         return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
-    CSX_HIP(timed(stream, [&] { return csx::launch_pattern(elem_size, d_in, d_out, k, pattern, pattern_length, stream); }), "csx_pattern_compact");
-    if (d_count) CSX_HIP(csx::launch_store_u64(d_count, k, stream), "csx_pattern_compact count");
+    // tile size per width, measured at 2^29 on two 32-bit patterns: 4 vectors
+    // per thread for 1- and 2-byte elements, 2 for 4- and 8-byte
+    const uint32_t vpt = elem_size <= 2 ? 4 : 2;
+    const uint64_t te = 256ull * vpt * (16 / elem_size);
+    const uint64_t tiles = (n + te - 1) / te;
+    if (!launch_fits(tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)tiles);
+    if (k == 0) {
+        if (d_count) CSX_HIP(csx::launch_store_u64(d_count, 0, stream), "csx_pattern_compact count");
+    }
+    else {
+        // the tile engine with mask words generated from the pattern and each
+        // tile's offset in closed form: one kernel, no mask, no scan
+        csx::TileArgs a{};
+        a.in = static_cast<const uint8_t*>(d_in);
+        a.out = static_cast<uint8_t*>(d_out);
+        a.n = n;
+        a.count_out = d_count;
+        a.pat_len = pattern_length;
+        for (uint32_t b = 0; b < 64; b++)
+            if ((pattern >> (31 - b % pattern_length)) & 1u) a.pat_bits |= 1ull << b;
+        for (uint32_t p = 0; p < pattern_length; p++) a.pat_pop += (pattern >> (31 - p)) & 1u;
+        const bool aligned = ((uintptr_t)d_in & 15) == 0;
+        CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModePattern, true, CSX_MSB_FIRST, aligned, a, stream, (int)vpt); }),
+                "csx_pattern_compact");
+    }
     if (h_count) *h_count = k;
     return ok();
 }

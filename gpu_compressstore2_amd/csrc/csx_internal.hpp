@@ -52,7 +52,6 @@ hipError_t launch_gen_values(void* out, uint64_t n, uint32_t w, int kind, uint64
 hipError_t launch_predicate(const void* in, uint64_t n, const PredSpec& p, uint8_t* mask, int order, uint32_t* counts, uint64_t* total, WsHeader* hdr,
                             uint64_t* status, uint64_t init_tiles, hipStream_t s);
 // periodic-pattern compaction (csx_pattern.hip); k = selected count
-hipError_t launch_pattern(uint32_t w, const void* in, void* out, uint64_t k, uint32_t pattern, uint32_t L, hipStream_t s);
 hipError_t launch_validate(const void* a, const void* b, uint64_t len, uint32_t w, uint64_t* failures, hipStream_t s);
 
 // tile engine, instantiated once per element width (csx_tile.hip)

@@ -259,6 +259,15 @@ template <int W> __device__ __forceinline__ uint64_t load_elem(const uint8_t* p)
     return (uint64_t)*reinterpret_cast<const T*>(p);
 }
 
+// Selected elements before element e under a periodic pattern: whole periods
+// times their count, plus the pattern's first (e mod L) bits.
+__device__ __forceinline__ uint64_t pattern_offset(const TileArgs& a, uint64_t e)
+{
+    const uint64_t q = e / a.pat_len;
+    const uint32_t r = (uint32_t)(e - q * a.pat_len);
+    return q * a.pat_pop + (uint64_t)__builtin_popcountll(a.pat_bits & ((1ull << r) - 1ull));
+}
+
 // One tile: every thread of the block calls this with the same `tile`.
 // Contains __syncthreads; LDS in `sh` may be reused once all threads return
 // and pass a barrier.
@@ -306,12 +315,26 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
     uint32_t w[4] = {0, 0, 0, 0};
     uint32_t wsum = 0;
     if (t < (uint32_t)C::MASK_THREADS) {
-        const uint64_t off = tbase / 8 + 16ull * t;
-        u32x4 x = load_mask16<ALIGNED>(a.mask, off, nbytes);
-        w[0] = to_lsb_word<ORDER>(x.x);
-        w[1] = to_lsb_word<ORDER>(x.y);
-        w[2] = to_lsb_word<ORDER>(x.z);
-        w[3] = to_lsb_word<ORDER>(x.w);
+        if constexpr (MODE == kModePattern) {
+            // 32 elements from phase (e mod L) of the period: bits [phase, phase + 32)
+            // of the unrolled pattern (phase < L <= 32)
+            const uint64_t e0 = tbase + 128ull * t;
+            uint32_t phase = (uint32_t)(e0 % a.pat_len);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                w[k] = (uint32_t)(a.pat_bits >> phase);
+                phase += 32 % a.pat_len;
+                if (phase >= a.pat_len) phase -= a.pat_len;
+            }
+        }
+        else {
+            const uint64_t off = tbase / 8 + 16ull * t;
+            u32x4 x = load_mask16<ALIGNED>(a.mask, off, nbytes);
+            w[0] = to_lsb_word<ORDER>(x.x);
+            w[1] = to_lsb_word<ORDER>(x.y);
+            w[2] = to_lsb_word<ORDER>(x.z);
+            w[3] = to_lsb_word<ORDER>(x.w);
+        }
         if (partial) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -344,6 +367,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         if (tile_count == 0) {
             if (MODE == kModeTwoPass && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = prefix_total;
             if (MODE == kModeGiven && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = a.given_off;
+            if (MODE == kModePattern && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = pattern_offset(a, tbase);
             return;
         }
     }
@@ -398,6 +422,13 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         if (t == 0) {
             sh.off = a.given_off;
             if (tile == ntiles - 1 && a.count_out) *a.count_out = a.given_off + tile_count;
+        }
+        __syncthreads();
+    }
+    else if constexpr (MODE == kModePattern) {
+        if (t == 0) {
+            sh.off = pattern_offset(a, tbase);
+            if (tile == ntiles - 1 && a.count_out) *a.count_out = sh.off + tile_count;
         }
         __syncthreads();
     }

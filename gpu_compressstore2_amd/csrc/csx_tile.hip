@@ -15,6 +15,12 @@ template <int W, int ORDER, bool ALIGNED, int VPT> static void launch_writeout_v
     k_tile<W, ORDER, ALIGNED, kModeWriteout, true, VPT><<<(unsigned)tiles, 256, 0, s>>>(a);
 }
 
+template <int W, bool ALIGNED, int VPT> static void launch_pattern_vpt(const TileArgs& a, hipStream_t s)
+{
+    const uint64_t tiles = (a.n + TileCfg<W, VPT>::TE - 1) / TileCfg<W, VPT>::TE;
+    k_tile<W, kBitMsb, ALIGNED, kModePattern, true, VPT><<<(unsigned)tiles, 256, 0, s>>>(a);
+}
+
 template <int W, int ORDER, bool ALIGNED>
 static void launch_tile_impl(int mode, bool staged, int vpt, const TileArgs& a, unsigned grid, hipStream_t s)
 {
@@ -27,6 +33,16 @@ static void launch_tile_impl(int mode, bool staged, int vpt, const TileArgs& a, 
     else if (mode == kModeGiven) {
         // one tile at a.given_off (inputs of at most one tile: the whole call)
         k_tile<W, ORDER, ALIGNED, kModeGiven, true><<<grid, 256, 0, s>>>(a);
+    }
+    else if (mode == kModePattern) {
+        // no mask to order: one instantiation per tile size
+        if constexpr (ORDER == kBitMsb) {
+            switch (vpt) {
+                case 2: launch_pattern_vpt<W, ALIGNED, 2>(a, s); break;
+                case 4: launch_pattern_vpt<W, ALIGNED, 4>(a, s); break;
+                default: launch_pattern_vpt<W, ALIGNED, 8>(a, s); break;
+            }
+        }
     }
     else if (staged) {
         // three-phase writeout: a tile's offset comes from the chunk offsets,

@@ -23,7 +23,9 @@ template <int W> constexpr uint64_t kTileElems = 256ull * kTileVpt<W> * (16ull /
 constexpr uint64_t kScanTileElems = 256ull * 16ull;
 
 // kModeGiven: the tile's output offset is supplied in TileArgs::given_off
-enum TileMode { kModeWriteout = 0, kModeFused = 1, kModeTwoPass = 2, kModeGiven = 3 };
+// kModePattern: no mask in memory; mask words and the tile's offset follow
+// from a periodic pattern (TileArgs::pat_*)
+enum TileMode { kModeWriteout = 0, kModeFused = 1, kModeTwoPass = 2, kModeGiven = 3, kModePattern = 4 };
 
 // Arguments of one tile-engine launch.
 struct TileArgs {
@@ -47,6 +49,12 @@ struct TileArgs {
     uint32_t log2g;
     // given offset (kModeGiven)
     uint64_t given_off;
+    // periodic pattern (kModePattern): bit k of pat_bits (LSB first) is the
+    // selection of element k for k < 64, a period of pat_len elements with
+    // pat_pop selected
+    uint64_t pat_bits;
+    uint32_t pat_len;
+    uint32_t pat_pop;
 };
 
 // One column of a batched compaction (device-resident table).

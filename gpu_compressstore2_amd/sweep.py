@@ -1,6 +1,9 @@
-"""Per-phase sweep of the single-column compaction (2^28 float32 by default).
+"""Sweeps of the single-column compaction.
 
     python -m gpu_compressstore2_amd.sweep [--log2n 28] [--reps 10] [--json FILE]
+    python -m gpu_compressstore2_amd.sweep --widths [--log2n 29] [--reps 10] [--json FILE]
+
+Default: the per-phase sweep over 2^28 float32.
 
 Masks: Bernoulli at 1, 10, 50, 90 and 99 % selectivity, and alternating
 1-runs / 0-runs with geometric lengths of mean 256 (50 %).  Chunk lengths 256,
@@ -38,12 +41,74 @@ def median_ms(fn, reps):
     return times[len(times) // 2]
 
 
+WIDTHS = (("u8", 1, csx.VALUES_INTEGER), ("u16", 2, csx.VALUES_INTEGER), ("u32", 4, csx.VALUES_INTEGER), ("u64", 8, csx.VALUES_INTEGER),
+          ("f64", 8, csx.VALUES_FLOAT))
+
+
+def widths_sweep(args):
+    """Element widths and mask shapes: 2^29 elements (default) of u8, u16, u32,
+    u64 and f64, each with a Bernoulli(0.25) mask, a periodic seeded 32-bit
+    pattern and Zipf(s = 1.2) per-4096-block selectivity over {0, 1/64, ..., 1}.
+    Per row: the default three-phase call, the tuned call (selectivity = k/n)
+    and, for the periodic mask, the mask-free pattern kernel; each the median
+    of `reps` HIP-event timings, its count checked against the first call's."""
+    n = 1 << args.log2n
+    pattern = int(csx.gen_values(1, 4, seed=29).view(torch.int32).item()) & 0xFFFFFFFF  # a seeded 32-bit pattern
+    masks = {
+        "bernoulli_0.25": csx.gen_mask_bernoulli(n, 0.25, seed=3),
+        "periodic": csx.gen_mask_pattern(n, pattern, 32),
+        "zipf_blocks": csx.gen_mask_zipf_blocks(n, 4096, 1.2, 65, seed=5),
+    }
+    count = torch.empty(1, dtype=torch.int64, device="cuda")
+    rows = []
+    print("type;mask;selected;default_ms;default_GB/s;tuned_ms;tuned_GB/s;tuned_config;pattern_kernel_ms", flush=True)
+    for name, w, kind in WIDTHS:
+        data = csx.gen_values(n, w, kind, seed=100 + w + kind)
+        out = torch.empty_like(data)
+        ws = csx.alloc_workspace(csx.auto_workspace_size(n, w))
+        for mname, mask in masks.items():
+            _, k = csx.compact(data, mask, n=n, out=out, workspace=ws)
+
+            def checked(fn):
+                t = median_ms(fn, args.reps)
+                if int(count.item()) != k:
+                    print(f"count mismatch {name} {mname}", file=sys.stderr)
+                    sys.exit(1)
+                return t
+
+            td = checked(lambda: csx.compact(data, mask, n=n, out=out, count=count, workspace=ws, host_count=False))
+            ta = checked(lambda: csx.compact_auto(data, mask, k / n, n=n, out=out, count=count, workspace=ws, host_count=False))
+            cfg = csx.tune_lookup(w, n, k / n).as_dict()
+            tp = None
+            if mname == "periodic":
+                tp = checked(lambda: csx.pattern_compact(data, pattern, 32, n=n, out=out, count=count))
+            b = n * w + n / 8 + k * w
+            row = {"type": name, "mask": mname, "selected": k, "default_ms": round(td, 5), "default_GB/s": round(b / (td * 1e6), 1),
+                   "tuned_ms": round(ta, 5), "tuned_GB/s": round(b / (ta * 1e6), 1), "tuned_config": cfg,
+                   "pattern_kernel_ms": None if tp is None else round(tp, 5)}
+            rows.append(row)
+            print(f"{name};{mname};{k};{td:.5f};{row['default_GB/s']};{ta:.5f};{row['tuned_GB/s']};{cfg};{'' if tp is None else f'{tp:.5f}'}",
+                  flush=True)
+        del data, out, ws
+        torch.cuda.empty_cache()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--log2n", type=int, default=28)
+    ap.add_argument("--log2n", type=int, default=None, help="log2 of the element count (28; 29 with --widths)")
     ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--widths", action="store_true", help="the element-width and mask-shape sweep instead of the per-phase sweep")
     ap.add_argument("--json", default=None, help="also write the rows as JSON")
     args = ap.parse_args()
+    if args.widths:
+        args.log2n = 29 if args.log2n is None else args.log2n
+        rows = widths_sweep(args)
+        if args.json:
+            with open(args.json, "w") as f:
+                json.dump(rows, f, indent=1)
+        return
+    args.log2n = 28 if args.log2n is None else args.log2n
     n = 1 << args.log2n
     g = torch.Generator(device="cuda").manual_seed(11)
     data = torch.rand(n, device="cuda", generator=g)
