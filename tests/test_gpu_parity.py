@@ -351,6 +351,15 @@ def test_pattern_compact(O, w):
     expect, ke = O.compact(data, mask, n)
     out, k = csx.pattern_compact(dev(data), pat, 8)
     assert k == ke and (host(out, NP[w])[:k] == expect).all()
+    # input not 16-byte aligned (element loads), output offset too
+    if w < 8:
+        d = dev(data)
+        for off in (1, 3):
+            mask, _ = O.mask_pattern((n - off + 7) // 8, pat, 8)
+            expect, ke = O.compact(data[off:], mask, n - off)
+            d_out = torch.zeros(n + 8, dtype=d.dtype, device="cuda")[off:]
+            _, k = csx.pattern_compact(d[off:], pat, 8, out=d_out)
+            assert k == ke and (host(d_out, NP[w])[:k] == expect).all(), off
 
 
 @pytest.mark.parametrize("nstreams", [1, 2, 3, 8])
