@@ -85,38 +85,83 @@ __device__ __forceinline__ uint32_t find_segment(const uint64_t* tile_prefix, ui
     return lo;
 }
 
+// Each wave counts a contiguous run of tiles: one segment search for its first
+// tile, then the segment advances as the run crosses segment ends (empty
+// segments included), and kCountUnroll tiles' mask loads are in flight together.
+constexpr int kCountUnroll = 4;
+
 template <int W, int ORDER, int VPT>
 __global__ __launch_bounds__(256) void k_batch_counts(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, uint32_t* tcounts)
 {
     constexpr uint64_t TE = TileCfg<W, VPT>::TE;
     constexpr uint32_t VEC_PER_TILE = (uint32_t)(TE / 128);
+    constexpr uint32_t VPL = (VEC_PER_TILE + kWave - 1) / kWave;  // mask vectors per lane per tile
     const uint32_t lane = lane_id();
     const uint64_t total = tile_prefix[nseg];
     const uint64_t wave = (blockIdx.x * 256ull + threadIdx.x) / kWave;
     const uint64_t waves = gridDim.x * (256ull / kWave);
-    for (uint64_t g = wave; g < total; g += waves) {
-        const uint32_t s = find_segment(tile_prefix, nseg, g);
-        const Segment seg = segs[s];
-        const uint64_t lt = g - tile_prefix[s];
-        const uint64_t nbytes = (seg.n + 7) / 8;
-        const bool aligned = ((uintptr_t)seg.mask & 15) == 0;
-        uint32_t c = 0;
-        for (uint32_t v = lane; v < VEC_PER_TILE; v += kWave) {
-            const uint64_t vec = lt * VEC_PER_TILE + v;
-            const u32x4 x = aligned ? load_mask16<true>(seg.mask, vec * 16, nbytes) : load_mask16<false>(seg.mask, vec * 16, nbytes);
-            uint32_t w[4] = {to_lsb_word<ORDER>(x.x), to_lsb_word<ORDER>(x.y), to_lsb_word<ORDER>(x.z), to_lsb_word<ORDER>(x.w)};
-            const uint64_t e0 = vec * 128;
-            if (e0 + 128 > seg.n) {
+    const uint64_t per = (total + waves - 1) / waves;
+    const uint64_t g0 = wave * per;
+    const uint64_t g1 = g0 + per < total ? g0 + per : total;
+    if (g0 >= g1) return;  // wave-uniform
+    uint32_t s = find_segment(tile_prefix, nseg, g0);
+    uint64_t s_first = tile_prefix[s], s_end = tile_prefix[s + 1];
+    Segment seg = segs[s];
+    for (uint64_t g = g0; g < g1; g += kCountUnroll) {
+        u32x4 x[kCountUnroll][VPL];
+        uint64_t n_of[kCountUnroll], e_of[kCountUnroll];
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const uint64_t b = e0 + 32 * k;
-                    w[k] = b >= seg.n ? 0u : low_bits(w[k], (uint32_t)(seg.n - b));
+        for (int u = 0; u < kCountUnroll; u++) {
+            const uint64_t gu = g + u;
+            n_of[u] = 0;
+            e_of[u] = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < VPL; q++) x[u][q] = u32x4{0, 0, 0, 0};
+            if (gu >= g1) continue;
+            while (gu >= s_end) {  // next non-empty segment
+                s++;
+                s_first = s_end;
+                s_end = tile_prefix[s + 1];
+                seg = segs[s];
+            }
+            const uint64_t lt = gu - s_first;
+            const uint64_t nbytes = (seg.n + 7) / 8;
+            const bool aligned = ((uintptr_t)seg.mask & 15) == 0;
+            n_of[u] = seg.n;
+            e_of[u] = lt * TE;
+#pragma unroll
+            for (uint32_t q = 0; q < VPL; q++) {
+                const uint32_t v = q * kWave + lane;
+                if (v < VEC_PER_TILE) {
+                    const uint64_t vec = lt * VEC_PER_TILE + v;
+                    x[u][q] = aligned ? load_mask16<true>(seg.mask, vec * 16, nbytes) : load_mask16<false>(seg.mask, vec * 16, nbytes);
                 }
             }
-            c += popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
         }
-        c = wave_sum(c);
-        if (lane == 0) tcounts[g] = c;
+#pragma unroll
+        for (int u = 0; u < kCountUnroll; u++) {
+            const uint64_t gu = g + u;
+            if (gu >= g1) break;
+            uint32_t c = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < VPL; q++) {
+                const uint32_t v = q * kWave + lane;
+                uint32_t w[4] = {to_lsb_word<ORDER>(x[u][q].x), to_lsb_word<ORDER>(x[u][q].y), to_lsb_word<ORDER>(x[u][q].z),
+                                 to_lsb_word<ORDER>(x[u][q].w)};
+                const uint64_t e0 = e_of[u] + 128ull * v;
+                if (v >= VEC_PER_TILE) continue;
+                if (e0 + 128 > n_of[u]) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const uint64_t b = e0 + 32 * k;
+                        w[k] = b >= n_of[u] ? 0u : low_bits(w[k], (uint32_t)(n_of[u] - b));
+                    }
+                }
+                c += popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
+            }
+            c = wave_sum(c);
+            if (lane == 0) tcounts[gu] = c;
+        }
     }
 }
 
