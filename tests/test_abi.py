@@ -94,6 +94,42 @@ def test_null_data_pointers_rejected():
 def test_unknown_pipeline_rejected():
     need = csx.workspace_size(100, 4, 1024)
     assert csx.lib().csx_compact(None, None, None, 100, 4, 1024, 0, 9, None, None, ctypes.c_void_p(4096), need, None) == 1
+    # the highest known pipeline (sparse) passes the pipeline check and stops at the null pointers;
+    # one past it is unknown
+    L = csx.lib()
+    assert L.csx_compact(None, None, None, 100, 4, 1024, 0, csx.PIPELINE_SPARSE, None, None, ctypes.c_void_p(4096), need, None) == 1
+    assert b"null" in L.csx_last_error()
+    assert L.csx_compact(None, None, None, 100, 4, 1024, 0, csx.PIPELINE_SPARSE + 1, None, None, ctypes.c_void_p(4096), need, None) == 1
+    assert b"unknown pipeline" in L.csx_last_error()
+
+
+def test_writeout_variant_range():
+    L = csx.lib()
+    # the sparse variant is known: rejected for the null pointers, not the variant
+    assert L.csx_writeout(None, None, None, None, None, 10, 4, 32, 0, csx.WRITEOUT_SPARSE, None) == 1
+    assert b"null" in L.csx_last_error()
+    assert L.csx_writeout(None, None, None, None, None, 10, 4, 32, 0, csx.WRITEOUT_SPARSE + 1, None) == 1
+    assert b"unknown writeout variant" in L.csx_last_error()
+
+
+def test_tuning_table_accepts_sparse_and_rejects_bad_rows(tmp_path):
+    good = tmp_path / "t.txt"
+    good.write_text("# csx tuning table 1\ncompact 4 20 0.010000 3 4096 8 0.01\n")
+    csx.tune_clear()
+    try:
+        csx.tune_load(str(good))
+        c = csx.tune_lookup(4, 1 << 20, 0.01)
+        assert (c.pipeline, c.chunk_len, c.vpt) == (csx.PIPELINE_SPARSE, 4096, 8)
+        for bad in ("compact 4 20 0.01 3 4096 2 0.01\n",  # sparse takes no tile size but 8
+                    "compact 4 20 0.01 4 1024 8 0.01\n",  # unknown pipeline
+                    "compact 4 20 1.5 1 1024 8 0.01\n"):  # selectivity out of range
+            f = tmp_path / "bad.txt"
+            f.write_text(bad)
+            with pytest.raises(csx.CsxError):
+                csx.tune_load(str(f))
+    finally:
+        csx.tune_clear()
+        csx.tune_load(csx.TUNED_TABLE)
 
 
 def test_workspace_size_grows_with_n_and_shrinking_chunks():
