@@ -172,7 +172,8 @@ __global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32
     using C = TileCfg<W, VPT>;
     __shared__ TileShared<C> sh;
     __shared__ uint32_t s_seg;
-    const uint64_t g = blockIdx.x;
+    // 4- and 8-byte columns: each XCD takes a contiguous run of tiles (see k_tile)
+    const uint64_t g = W >= 4 ? xcd_contiguous(blockIdx.x, gridDim.x) : (uint64_t)blockIdx.x;
     if (g >= tile_prefix[nseg]) return;  // surplus of the host's bound: uniform exit
     if (threadIdx.x < kWave) {
         const uint32_t s0 = find_segment(tile_prefix, nseg, g);  // wave 0 together

@@ -542,7 +542,7 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a)
         __syncthreads();
         tile = sh.tile;
     }
-    else if constexpr (MODE == kModeWriteout && W >= 4) {
+    else if constexpr ((MODE == kModeWriteout || MODE == kModePattern) && W >= 4) {
         // measured faster for 4- and 8-byte elements at 50-99 % selectivity (2-6 %),
         // slower for 1- and 2-byte ones; the grid is exactly one block per tile
         tile = xcd_contiguous(blockIdx.x, gridDim.x);
