@@ -172,8 +172,8 @@ __global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32
     using C = TileCfg<W, VPT>;
     __shared__ TileShared<C> sh;
     __shared__ uint32_t s_seg;
-    // 4- and 8-byte columns: each XCD takes a contiguous run of tiles (see k_tile)
-    const uint64_t g = W >= 4 ? xcd_contiguous(blockIdx.x, gridDim.x) : (uint64_t)blockIdx.x;
+    // runs of consecutive tiles per XCD (see xcd_runs)
+    const uint64_t g = xcd_runs(blockIdx.x, gridDim.x);
     if (g >= tile_prefix[nseg]) return;  // surplus of the host's bound: uniform exit
     if (threadIdx.x < kWave) {
         const uint32_t s0 = find_segment(tile_prefix, nseg, g);  // wave 0 together

@@ -519,16 +519,20 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
 
 // One tile per workgroup: tile = blockIdx.x (writeout) or a ticket (fused).
 // The workgroups of a launch go to the chip's 8 XCDs round-robin (block b to
-// XCD b mod 8), and each XCD has its own L2.  Block b -> tile
-// (b mod 8) * ceil-or-floor share + b / 8 gives each XCD one contiguous run of
-// tiles (a bijection on [0, n); the first n mod 8 XCDs take one tile more), so
-// the output line two neighbouring tiles share is completed in one L2 rather
-// than written back half by each of two.
+// XCD b mod 8), and each XCD has its own L2.  Blocks are taken in groups of
+// 8 * kXcdRun: within a group, XCD x gets the run of kXcdRun consecutive tiles
+// starting at x * kXcdRun, so the output line two neighbouring tiles share is
+// mostly completed in one L2 instead of being written back half by each of
+// two, while the chip as a whole still sweeps the column front to back.  A
+// ragged last group keeps the plain order, so the map is a bijection on [0, n).
 constexpr uint64_t kXcds = 8;
-__device__ __forceinline__ uint64_t xcd_contiguous(uint64_t b, uint64_t n)
+constexpr uint64_t kXcdRun = 16;
+__device__ __forceinline__ uint64_t xcd_runs(uint64_t b, uint64_t n)
 {
-    const uint64_t x = b % kXcds, i = b / kXcds, base = n / kXcds, rem = n % kXcds;
-    return x < rem ? x * (base + 1) + i : rem * (base + 1) + (x - rem) * base + i;
+    constexpr uint64_t group = kXcds * kXcdRun;
+    const uint64_t g = b / group;
+    if ((g + 1) * group > n) return b;
+    return g * group + (b % kXcds) * kXcdRun + (b / kXcds) % kXcdRun;
 }
 
 template <int W, int ORDER, bool ALIGNED, int MODE, bool STAGED, int VPT = kTileVpt<W>>
@@ -542,10 +546,9 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a)
         __syncthreads();
         tile = sh.tile;
     }
-    else if constexpr ((MODE == kModeWriteout || MODE == kModePattern) && W >= 4) {
-        // measured faster for 4- and 8-byte elements at 50-99 % selectivity (2-6 %),
-        // slower for 1- and 2-byte ones; the grid is exactly one block per tile
-        tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    else if constexpr (MODE == kModeWriteout || MODE == kModePattern) {
+        // the grid is exactly one block per tile
+        tile = xcd_runs(blockIdx.x, gridDim.x);
     }
     else {
         tile = blockIdx.x;

@@ -10,6 +10,10 @@
 //   mix2:1  1 GiB read + 0.5 GiB written: the byte mix of a 50 % compaction
 // The best "mix2:1" figure is the practical ceiling of the compaction's
 // writeout; the published 8 TB/s is the roofline's peak.  Median of 9 runs.
+// The one-pass grids also run with blocks taken in XCD runs: the workgroups
+// of a launch go to the 8 XCDs round-robin, and block b -> block
+// (b / 8R) * 8R + (b mod 8) * R + (b / 8) mod R gives each XCD runs of R
+// consecutive blocks (the order of the compaction's writeout).
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdio>
@@ -18,13 +22,23 @@
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 #define CK(x) do { hipError_t e = (x); if (e != hipSuccess) { printf("err %s line %d\n", hipGetErrorString(e), __LINE__); return 1; } } while (0)
 
+// block order: plain (xr = 0) or runs of xr consecutive blocks per XCD
+__device__ __forceinline__ uint64_t blk_order(uint32_t xr)
+{
+    const uint64_t b = blockIdx.x, n = gridDim.x;
+    if (!xr) return b;
+    const uint64_t grp = 8ull * xr, g = b / grp;
+    if ((g + 1) * grp > n) return b;
+    return g * grp + (b % 8) * xr + (b / 8) % xr;
+}
+
 template <bool NT> __device__ __forceinline__ u32x4 ld(const u32x4* p) { if (NT) return __builtin_nontemporal_load(p); return *p; }
 template <bool NT> __device__ __forceinline__ void st(u32x4* p, u32x4 v) { if (NT) __builtin_nontemporal_store(v, p); else *p = v; }
 
-template <int U, bool NT> __global__ __launch_bounds__(256) void k_read(const u32x4* in, uint64_t nvec, uint32_t* sink)
+template <int U, bool NT> __global__ __launch_bounds__(256) void k_read(const u32x4* in, uint64_t nvec, uint32_t* sink, uint32_t xr)
 {
     u32x4 acc = {0, 0, 0, 0};
-    for (uint64_t b = (uint64_t)blockIdx.x * 256 * U + threadIdx.x; b < nvec; b += (uint64_t)gridDim.x * 256 * U) {
+    for (uint64_t b = blk_order(xr) * 256 * U + threadIdx.x; b < nvec; b += (uint64_t)gridDim.x * 256 * U) {
         u32x4 v[U];
 #pragma unroll
         for (int j = 0; j < U; j++) { const uint64_t i = b + (uint64_t)j * 256; v[j] = i < nvec ? ld<NT>(in + i) : u32x4{0, 0, 0, 0}; }
@@ -33,15 +47,15 @@ template <int U, bool NT> __global__ __launch_bounds__(256) void k_read(const u3
     }
     if ((acc.x ^ acc.y ^ acc.z ^ acc.w) == 0x12345678u) sink[0] = 1;
 }
-template <int U, bool NT> __global__ __launch_bounds__(256) void k_write(u32x4* out, uint64_t nvec)
+template <int U, bool NT> __global__ __launch_bounds__(256) void k_write(u32x4* out, uint64_t nvec, uint32_t xr)
 {
-    for (uint64_t b = (uint64_t)blockIdx.x * 256 * U + threadIdx.x; b < nvec; b += (uint64_t)gridDim.x * 256 * U)
+    for (uint64_t b = blk_order(xr) * 256 * U + threadIdx.x; b < nvec; b += (uint64_t)gridDim.x * 256 * U)
 #pragma unroll
         for (int j = 0; j < U; j++) { const uint64_t i = b + (uint64_t)j * 256; if (i < nvec) st<NT>(out + i, u32x4{(uint32_t)i, 1, 2, 3}); }
 }
-template <int U, bool NT> __global__ __launch_bounds__(256) void k_copy(const u32x4* in, u32x4* out, uint64_t nvec)
+template <int U, bool NT> __global__ __launch_bounds__(256) void k_copy(const u32x4* in, u32x4* out, uint64_t nvec, uint32_t xr)
 {
-    for (uint64_t b = (uint64_t)blockIdx.x * 256 * U + threadIdx.x; b < nvec; b += (uint64_t)gridDim.x * 256 * U) {
+    for (uint64_t b = blk_order(xr) * 256 * U + threadIdx.x; b < nvec; b += (uint64_t)gridDim.x * 256 * U) {
         u32x4 v[U];
 #pragma unroll
         for (int j = 0; j < U; j++) { const uint64_t i = b + (uint64_t)j * 256; v[j] = i < nvec ? ld<NT>(in + i) : u32x4{0, 0, 0, 0}; }
@@ -50,10 +64,10 @@ template <int U, bool NT> __global__ __launch_bounds__(256) void k_copy(const u3
     }
 }
 // out vector o (o < nvec/2) <- in[2o] ^ in[2o+1]; two reads per write, reads of a wave contiguous
-template <int U, bool NT> __global__ __launch_bounds__(256) void k_mix(const u32x4* in, u32x4* out, uint64_t nvec)
+template <int U, bool NT> __global__ __launch_bounds__(256) void k_mix(const u32x4* in, u32x4* out, uint64_t nvec, uint32_t xr)
 {
     const uint64_t nout = nvec / 2;
-    for (uint64_t b = (uint64_t)blockIdx.x * 256 * U + threadIdx.x; b < nout; b += (uint64_t)gridDim.x * 256 * U) {
+    for (uint64_t b = blk_order(xr) * 256 * U + threadIdx.x; b < nout; b += (uint64_t)gridDim.x * 256 * U) {
         u32x4 a[U], c[U];
 #pragma unroll
         for (int j = 0; j < U; j++) {
@@ -89,14 +103,18 @@ template <int U, bool NT> int run(u32x4* in, u32x4* out, uint32_t* sink, uint64_
     for (unsigned g : {1024u, 2048u, 4096u, 8192u, 16384u, 0u}) {
         unsigned gr = g ? g : (unsigned)((nvec + 256ull * U - 1) / (256ull * U));
         unsigned grm = g ? g : (unsigned)((nvec / 2 + 256ull * U - 1) / (256ull * U));
-        float r = timeit([&] { k_read<U, NT><<<gr, 256>>>(in, nvec, sink); }, e0, e1);
-        float w = timeit([&] { k_write<U, NT><<<gr, 256>>>(out, nvec); }, e0, e1);
-        float c = timeit([&] { k_copy<U, NT><<<gr, 256>>>(in, out, nvec); }, e0, e1);
-        float m = timeit([&] { k_mix<U, NT><<<grm, 256>>>(in, out, nvec); }, e0, e1);
-        CK(hipGetLastError());
-        printf("U=%d NT=%d grid=%6u  read %7.1f  write %7.1f  copy %7.1f  mix2:1 %7.1f GB/s\n", U, (int)NT, gr, bytes / (r * 1e6), bytes / (w * 1e6),
-               2 * bytes / (c * 1e6), 1.5 * bytes / (m * 1e6));
-        fflush(stdout);
+        // XCD runs only where one block covers one fixed range (the one-pass grids)
+        for (uint32_t xr : {0u, 16u, 32u}) {
+            if (xr && g) continue;
+            float r = timeit([&] { k_read<U, NT><<<gr, 256>>>(in, nvec, sink, xr); }, e0, e1);
+            float w = timeit([&] { k_write<U, NT><<<gr, 256>>>(out, nvec, xr); }, e0, e1);
+            float c = timeit([&] { k_copy<U, NT><<<gr, 256>>>(in, out, nvec, xr); }, e0, e1);
+            float m = timeit([&] { k_mix<U, NT><<<grm, 256>>>(in, out, nvec, xr); }, e0, e1);
+            CK(hipGetLastError());
+            printf("U=%d NT=%d grid=%6u xcd-runs=%2u  read %7.1f  write %7.1f  copy %7.1f  mix2:1 %7.1f GB/s\n", U, (int)NT, gr, xr, bytes / (r * 1e6),
+                   bytes / (w * 1e6), 2 * bytes / (c * 1e6), 1.5 * bytes / (m * 1e6));
+            fflush(stdout);
+        }
     }
     return 0;
 }
