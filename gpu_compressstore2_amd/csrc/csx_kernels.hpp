@@ -259,12 +259,17 @@ template <int W> __device__ __forceinline__ uint64_t load_elem(const uint8_t* p)
     return (uint64_t)*reinterpret_cast<const T*>(p);
 }
 
-// Selected elements before element e under a periodic pattern: whole periods
-// times their count, plus the pattern's first (e mod L) bits.
-__device__ __forceinline__ uint64_t pattern_offset(const TileArgs& a, uint64_t e)
+// Selected elements before element e = tile * TE under a periodic pattern:
+// whole periods times their count, plus the pattern's first (e mod L) bits.
+// With tile = tq L + tr and TE = qT L + rT, e = (tile qT + tq rT) L + tr rT,
+// so only 32-bit divisions by L are needed (tr rT < 1024; a tile index fits
+// in 32 bits, the grid being one block per tile).
+template <uint32_t TE> __device__ __forceinline__ uint64_t pattern_offset(const TileArgs& a, uint64_t tile)
 {
-    const uint64_t q = e / a.pat_len;
-    const uint32_t r = (uint32_t)(e - q * a.pat_len);
+    const uint32_t L = a.pat_len, t32 = (uint32_t)tile;
+    const uint32_t qT = TE / L, rT = TE % L, tq = t32 / L, tr = t32 % L;
+    const uint64_t q = (uint64_t)t32 * qT + (uint64_t)tq * rT + (tr * rT) / L;
+    const uint32_t r = (tr * rT) % L;
     return q * a.pat_pop + (uint64_t)__builtin_popcountll(a.pat_bits & ((1ull << r) - 1ull));
 }
 
@@ -318,8 +323,10 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         if constexpr (MODE == kModePattern) {
             // 32 elements from phase (e mod L) of the period: bits [phase, phase + 32)
             // of the unrolled pattern (phase < L <= 32)
-            const uint64_t e0 = tbase + 128ull * t;
-            uint32_t phase = (uint32_t)(e0 % a.pat_len);
+            // (tile * TE + 128 t) mod L in 32-bit arithmetic: a tile index fits in
+            // 32 bits (the grid is one block per tile)
+            const uint32_t L = a.pat_len;
+            uint32_t phase = (((uint32_t)tile % L) * ((uint32_t)C::TE % L) + (128u * t) % L) % L;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 w[k] = (uint32_t)(a.pat_bits >> phase);
@@ -367,7 +374,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         if (tile_count == 0) {
             if (MODE == kModeTwoPass && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = prefix_total;
             if (MODE == kModeGiven && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = a.given_off;
-            if (MODE == kModePattern && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = pattern_offset(a, tbase);
+            if (MODE == kModePattern && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = pattern_offset<C::TE>(a, tile);
             return;
         }
     }
@@ -427,7 +434,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
     }
     else if constexpr (MODE == kModePattern) {
         if (t == 0) {
-            sh.off = pattern_offset(a, tbase);
+            sh.off = pattern_offset<C::TE>(a, tile);
             if (tile == ntiles - 1 && a.count_out) *a.count_out = sh.off + tile_count;
         }
         __syncthreads();
