@@ -525,23 +525,6 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
 }
 
 // One tile per workgroup: tile = blockIdx.x (writeout) or a ticket (fused).
-// The workgroups of a launch go to the chip's 8 XCDs round-robin (block b to
-// XCD b mod 8), and each XCD has its own L2.  Blocks are taken in groups of
-// 8 * kXcdRun: within a group, XCD x gets the run of kXcdRun consecutive tiles
-// starting at x * kXcdRun, so the output line two neighbouring tiles share is
-// mostly completed in one L2 instead of being written back half by each of
-// two, while the chip as a whole still sweeps the column front to back.  A
-// ragged last group keeps the plain order, so the map is a bijection on [0, n).
-constexpr uint64_t kXcds = 8;
-constexpr uint64_t kXcdRun = 16;
-__device__ __forceinline__ uint64_t xcd_runs(uint64_t b, uint64_t n)
-{
-    constexpr uint64_t group = kXcds * kXcdRun;
-    const uint64_t g = b / group;
-    if ((g + 1) * group > n) return b;
-    return g * group + (b % kXcds) * kXcdRun + (b / kXcds) % kXcdRun;
-}
-
 template <int W, int ORDER, bool ALIGNED, int MODE, bool STAGED, int VPT = kTileVpt<W>>
 __global__ __launch_bounds__(256) void k_tile(TileArgs a)
 {

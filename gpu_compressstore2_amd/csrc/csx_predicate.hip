@@ -65,7 +65,8 @@ __global__ __launch_bounds__(kPredBlock) void k_predicate(const T* __restrict__ 
     const uint64_t nbytes = (n + 7) / 8;
     const uint64_t chunks = (n + kPredChunk - 1) / kPredChunk;
     const uint32_t lane = lane_id();
-    const uint64_t wave0 = (blockIdx.x * (uint64_t)kPredBlock + threadIdx.x) / kWave;
+    // blocks in XCD runs (see xcd_runs): measured 4 % faster at 2^28 float32
+    const uint64_t wave0 = (xcd_runs(blockIdx.x, gridDim.x) * (uint64_t)kPredBlock + threadIdx.x) / kWave;
     const uint64_t waves = (uint64_t)gridDim.x * (kPredBlock / kWave);
     uint64_t selected = 0;
 
