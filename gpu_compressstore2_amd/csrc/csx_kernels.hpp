@@ -206,7 +206,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const uint32_t* counts, Out
 //     (vector j*256 + t) so every load instruction of a wave reads 1 KiB
 //     contiguous; a vector whose mask bits are all zero is not loaded;
 //   * the tile's output offset comes from the chunk offsets (three-phase
-//     writeout) or from a decoupled look-back over tile counts (fused);
+//     writeout), a decoupled look-back over tile counts (fused), the tile
+//     and group counts (two-pass), the caller (given) or, with no mask in
+//     memory, a periodic pattern whose mask words the tile generates;
 //   * STAGED: selected elements are written to LDS at their rank, shifted so
 //     LDS byte p maps to a 16-byte-aligned global address, then stored with
 //     16-byte stores (element stores only in the two boundary vectors that
@@ -229,7 +231,6 @@ template <typename C> struct TileShared {
     uint64_t scr[C::BLOCK / kWave];
     uint64_t tile;
     uint64_t off;
-    uint32_t any;
 };
 
 template <int W> __device__ __forceinline__ uint64_t vec_elem(const u32x4& x, int k)
@@ -544,34 +545,6 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a)
         tile = blockIdx.x;
     }
     tile_body<W, ORDER, ALIGNED, MODE, STAGED, VPT>(a, sh, tile);
-}
-
-// Persistent form: a grid sized to the machine walks all tiles, round-robin
-// (writeout) or by ticket (fused); the loop ends for every wave once the
-// tile index passes the last tile.
-template <int W, int ORDER, bool ALIGNED, int MODE, bool STAGED, int VPT = kTileVpt<W>>
-__global__ __launch_bounds__(256) void k_tile_loop(TileArgs a)
-{
-    using C = TileCfg<W, VPT>;
-    __shared__ TileShared<C> sh;
-    const uint64_t ntiles = (a.n + C::TE - 1) / C::TE;
-    if constexpr (MODE == kModeFused) {
-        while (true) {
-            if (threadIdx.x == 0) sh.tile = atomicAdd(&a.hdr->ticket, 1u);
-            __syncthreads();
-            const uint64_t tile = sh.tile;
-            __syncthreads();
-            if (tile >= ntiles) break;
-            tile_body<W, ORDER, ALIGNED, MODE, STAGED, VPT>(a, sh, tile);
-            __syncthreads();
-        }
-    }
-    else {
-        for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-            tile_body<W, ORDER, ALIGNED, MODE, STAGED, VPT>(a, sh, tile);
-            __syncthreads();
-        }
-    }
 }
 
 // ======================================================================
