@@ -1169,7 +1169,12 @@ csx_status csx_compact_predicate(const void* d_in, uint64_t n, const csx_predica
         a.counts = counts;
         a.log2c = log2u(kPredChunk);
         const bool aligned = (((uintptr_t)d_in | (uintptr_t)mask) & 15) == 0;
-        CSX_HIP(timed(stream, [&] { return dispatch_tile(w, csx::kModeWriteout, true, bit_order, aligned, a, stream); }),
+        // the writeout's tile size: the tuned three-phase choice for this width
+        // and size at unknown selectivity (the chunk offsets serve any tile size)
+        uint32_t vpt = 8;
+        const csx_config tc = lookup_config(w, n, -1.0);
+        if (tc.pipeline == CSX_PIPELINE_THREE_PHASE && launch_fits((n + 256ull * tc.vpt * (16 / w) - 1) / (256ull * tc.vpt * (16 / w)))) vpt = tc.vpt;
+        CSX_HIP(timed(stream, [&] { return dispatch_tile(w, csx::kModeWriteout, true, bit_order, aligned, a, stream, (int)vpt); }),
                 "csx_compact_predicate writeout");
     }
     if (h_count) return finish_host_count(h_count, count_dst, hdr, stream);
