@@ -724,9 +724,12 @@ csx_status csx_pattern_compact(const void* d_in, void* d_out, uint64_t n, uint32
     if (k > 0 && (!d_in || !d_out)) return fail(CSX_ERR_INVALID, "null input or output pointer");
     if (((uintptr_t)d_in % elem_size) || ((uintptr_t)d_out % elem_size))
         return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
-    // tile size per width, measured at 2^29 on two 32-bit patterns: 4 vectors
-    // per thread for 1- and 2-byte elements, 2 for 4- and 8-byte
-    const uint32_t vpt = elem_size <= 2 ? 4 : 2;
+    // tile size, measured at 2^29 on 32-bit patterns of 12.5 % and 62.5 %
+    // density: 4 vectors per thread for 1- and 2-byte elements; for 4- and
+    // 8-byte ones 16 from half density up (4 % faster there), else 2 (2-7 % faster)
+    uint32_t pop = 0;
+    for (uint32_t p = 0; p < pattern_length; p++) pop += (pattern >> (31 - p)) & 1u;
+    const uint32_t vpt = elem_size <= 2 ? 4 : (2 * pop >= pattern_length ? 16 : 2);
     const uint64_t te = 256ull * vpt * (16 / elem_size);
     const uint64_t tiles = (n + te - 1) / te;
     if (!launch_fits(tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)tiles);

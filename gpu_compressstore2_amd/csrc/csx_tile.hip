@@ -40,6 +40,9 @@ static void launch_tile_impl(int mode, bool staged, int vpt, const TileArgs& a, 
             switch (vpt) {
                 case 2: launch_pattern_vpt<W, ALIGNED, 2>(a, s); break;
                 case 4: launch_pattern_vpt<W, ALIGNED, 4>(a, s); break;
+                case 16:
+                    if constexpr (W >= 2) launch_pattern_vpt<W, ALIGNED, 16>(a, s);
+                    break;
                 default: launch_pattern_vpt<W, ALIGNED, 8>(a, s); break;
             }
         }
