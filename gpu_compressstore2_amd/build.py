@@ -40,8 +40,9 @@ def _units():
     """(source, object, extra flags) for every translation unit."""
     units = [
         ("csx_core.hip", "csx_core.o", DEVICE),
-        ("csx_api.cpp", "csx_api.o", []),
-        ("csx_server.cpp", "csx_server.o", []),
+        # host code, compiled by hipcc as HIP: gfx950 only, no other device target
+        ("csx_api.cpp", "csx_api.o", [f"--offload-arch={ARCH}"]),
+        ("csx_server.cpp", "csx_server.o", [f"--offload-arch={ARCH}"]),
     ]
     for w in (1, 2, 4, 8):
         units.append(("csx_tile.hip", f"csx_tile_w{w}.o", DEVICE + [f"-DCSX_TILE_W={w}"]))
