@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstring>
 #include <algorithm>
 #include <mutex>
 #include <string>
@@ -1377,11 +1378,22 @@ csx_status csx_tune_load(const char* path)
     csx_status st = CSX_OK;
     while (fgets(line, sizeof line, f)) {
         lineno++;
-        if (line[0] == '#' || line[0] == '\n') continue;
+        const size_t len = strlen(line);
+        if (len == sizeof line - 1 && line[len - 1] != '\n' && !feof(f)) {
+            // a row never comes near the buffer; a split line would parse as two
+            st = fail(CSX_ERR_INVALID, "%s:%u: line longer than %zu bytes", path, lineno, sizeof line - 2);
+            break;
+        }
+        const char* row = line + strspn(line, " \t\r\n");  // blank lines and comments may be indented
+        if (*row == '#' || *row == '\0') continue;
         TuneCell c{};
         uint32_t w, vpt, cb;
-        if (sscanf(line, "compact %u %u %lf %d %u %u %lf", &c.w, &c.log2n, &c.sel, &c.cfg.pipeline, &c.cfg.chunk_len, &c.cfg.vpt, &c.ms) == 7) {
+        int end = -1, bend = -1;  // offsets past the row's fields and trailing blanks: the row must end there
+        if (sscanf(row, "compact %u %u %lf %d %u %u %lf %n", &c.w, &c.log2n, &c.sel, &c.cfg.pipeline, &c.cfg.chunk_len, &c.cfg.vpt, &c.ms,
+                   &end) == 7 &&
+            end >= 0 && row[end] == '\0') {
             if (check_width(c.w) || check_chunk(c.cfg.chunk_len) || check_vpt(c.cfg.vpt, c.w) || c.log2n > 40 || !(c.sel >= 0 && c.sel <= 1) ||
+                !(c.ms >= 0 && std::isfinite(c.ms)) ||
                 (c.cfg.pipeline != CSX_PIPELINE_THREE_PHASE && c.cfg.pipeline != CSX_PIPELINE_TWO_PASS && c.cfg.pipeline != CSX_PIPELINE_FUSED &&
                  c.cfg.pipeline != CSX_PIPELINE_SPARSE) ||
                 (c.cfg.pipeline != CSX_PIPELINE_THREE_PHASE && c.cfg.vpt != 8)) {
@@ -1390,7 +1402,7 @@ csx_status csx_tune_load(const char* path)
             }
             cells.push_back(c);
         }
-        else if (sscanf(line, "batch %u %u %u", &w, &vpt, &cb) == 3) {
+        else if (sscanf(row, "batch %u %u %u %n", &w, &vpt, &cb, &bend) == 3 && bend >= 0 && row[bend] == '\0') {
             if (check_width(w) || (vpt != 2 && vpt != 4 && vpt != 8) || cb == 0) {
                 st = fail(CSX_ERR_INVALID, "%s:%u: batch configuration out of range", path, lineno);
                 break;

@@ -132,6 +132,33 @@ def test_tuning_table_accepts_sparse_and_rejects_bad_rows(tmp_path):
         csx.tune_load(csx.TUNED_TABLE)
 
 
+def test_tuning_table_parse_is_strict_and_all_or_nothing(tmp_path):
+    f = tmp_path / "t.txt"
+    csx.tune_clear()
+    try:
+        # indented comments, CRLF endings, blank lines and a last row without a newline load
+        f.write_bytes(b"  # indented comment\r\n\r\n\tcompact 4 20 0.500000 1 256 4 0.02  \r\nbatch 4 4 2048")
+        csx.tune_load(str(f))
+        c = csx.tune_lookup(4, 1 << 20, 0.5)
+        assert (c.pipeline, c.chunk_len, c.vpt) == (csx.PIPELINE_THREE_PHASE, 256, 4)
+        ok_row = "compact 4 20 0.500000 0 4096 8 0.02\n"
+        for bad in ("compact 4 20 0.5 1 256 4 0.02 junk\n",  # trailing text after the row
+                    "compact 4 20 0.5 1 256 4 nan\n",  # time not a finite non-negative number
+                    "compact 4 20 0.5 1 256 4 -0.5\n",
+                    "compact 4 20 0.5 1 256 4 inf\n",
+                    "batch 4 4 2048 9\n",  # one field too many
+                    "compact 4 20 0.5 1 256 4 0.02" + " " * 600 + "compact 4 20 0.5 1 256 4 0.02\n"):  # over-long line
+            f.write_text(ok_row + bad)
+            with pytest.raises(csx.CsxError):
+                csx.tune_load(str(f))
+            # the good first row of a rejected file is not taken either
+            c = csx.tune_lookup(4, 1 << 20, 0.5)
+            assert (c.pipeline, c.chunk_len, c.vpt) == (csx.PIPELINE_THREE_PHASE, 256, 4)
+    finally:
+        csx.tune_clear()
+        csx.tune_load(csx.TUNED_TABLE)
+
+
 def test_workspace_size_grows_with_n_and_shrinking_chunks():
     a = csx.workspace_size(1 << 24, 4, 1024)
     b = csx.workspace_size(1 << 26, 4, 1024)
