@@ -2,6 +2,7 @@
 
     python -m gpu_compressstore2_amd.sweep [--log2n 28] [--reps 10] [--json FILE]
     python -m gpu_compressstore2_amd.sweep --widths [--log2n 29] [--reps 10] [--json FILE]
+    python -m gpu_compressstore2_amd.sweep --shapes [--log2n 28] [--reps 15] [--json FILE]
 
 Default: the per-phase sweep over 2^28 float32.
 
@@ -94,13 +95,79 @@ def widths_sweep(args):
     return rows
 
 
+def line_ratio(mask, n, w, p):
+    """Occupied 64-byte data lines (64 / w consecutive elements holding a
+    selected element) over the count a Bernoulli(p) mask occupies on average,
+    ceil(n / (64/w)) (1 - (1-p)^(64/w)); counted on the device with torch,
+    MSB-first bits, n a multiple of 64."""
+    epl = 64 // w
+    sh = torch.arange(7, -1, -1, dtype=torch.uint8, device=mask.device)
+    bits = ((mask[: n // 8].view(-1, 1) >> sh) & 1).view(-1, epl)
+    lines = int(bits.any(dim=1).sum().item())
+    expected = (n // epl) * (1.0 - (1.0 - p) ** epl)
+    return lines / expected if expected else 1.0
+
+
+def shapes_sweep(args):
+    """Tile size against mask shape: 2^28 elements (default) of every width;
+    Bernoulli masks at 10-90 %, alternating runs of geometric mean length
+    16-4096 (50 %) and Zipf(s = 1.2) per-4096-block selectivity; the
+    three-phase call (chunk 1024) with tiles of 2, 4, 8 and 16 vectors per
+    thread, and the two-pass call.  Each time the median of `reps` HIP-event
+    timings of the whole call; every count checked against the default
+    call's.  Per row also the mask's occupied-line ratio (line_ratio)."""
+    n = 1 << args.log2n
+    masks = {f"bernoulli_{p}": csx.gen_mask_bernoulli(n, p, seed=11) for p in (0.1, 0.25, 0.5, 0.75, 0.9)}
+    masks.update({f"runs_{r}": csx.gen_mask_runs(n, r, seed=7) for r in (16, 64, 256, 1024, 4096)})
+    masks["zipf_blocks"] = csx.gen_mask_zipf_blocks(n, 4096, 1.2, 65, seed=5)
+    forms = [("vpt2", csx.PIPELINE_THREE_PHASE, 2), ("vpt4", csx.PIPELINE_THREE_PHASE, 4), ("vpt8", csx.PIPELINE_THREE_PHASE, 8),
+             ("vpt16", csx.PIPELINE_THREE_PHASE, 16), ("two_pass", csx.PIPELINE_TWO_PASS, 8)]
+    count = torch.empty(1, dtype=torch.int64, device="cuda")
+    rows = []
+    print("width;mask;selected;line_ratio;" + ";".join(f"{f}_us" for f, _, _ in forms) + ";best", flush=True)
+    for w in (1, 2, 4, 8):
+        data = csx.gen_values(n, w, seed=3)
+        out = torch.empty_like(data)
+        ws = csx.alloc_workspace(csx.auto_workspace_size(n, w))
+        for _ in range(40):  # clocks up before the first timed row
+            csx.compact(data, masks["bernoulli_0.5"], n=n, out=out, workspace=ws, count=count, host_count=False)
+        for mname, mask in masks.items():
+            _, k = csx.compact(data, mask, n=n, out=out, workspace=ws)
+            row = {"width": w, "mask": mname, "selected": k, "line_ratio": round(line_ratio(mask, n, w, k / n), 3)}
+            for fname, pipeline, vpt in forms:
+                if vpt == 16 and w == 1:
+                    row[f"{fname}_us"] = None
+                    continue
+                cfg = csx.Config(pipeline, 1024, vpt, 0)
+                t = median_ms(lambda: csx.compact_config(data, mask, cfg, n=n, out=out, count=count, workspace=ws, host_count=False), args.reps)
+                if int(count.item()) != k:
+                    print(f"count mismatch {w} {mname} {fname}", file=sys.stderr)
+                    sys.exit(1)
+                row[f"{fname}_us"] = round(t * 1000, 1)
+            row["best"] = min((v, f) for f, v in row.items() if f.endswith("_us") and v is not None)[1][:-3]
+            rows.append(row)
+            print(f"{w};{mname};{k};{row['line_ratio']};" + ";".join("" if row[f'{f}_us'] is None else str(row[f'{f}_us']) for f, _, _ in forms)
+                  + f";{row['best']}", flush=True)
+        del data, out, ws
+        torch.cuda.empty_cache()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--log2n", type=int, default=None, help="log2 of the element count (28; 29 with --widths)")
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--widths", action="store_true", help="the element-width and mask-shape sweep instead of the per-phase sweep")
+    ap.add_argument("--shapes", action="store_true", help="tile size against mask shape, every width")
     ap.add_argument("--json", default=None, help="also write the rows as JSON")
     args = ap.parse_args()
+    if args.shapes:
+        args.log2n = 28 if args.log2n is None else args.log2n
+        rows = shapes_sweep(args)
+        if args.json:
+            with open(args.json, "w") as f:
+                json.dump(rows, f, indent=1)
+        return
     if args.widths:
         args.log2n = 29 if args.log2n is None else args.log2n
         rows = widths_sweep(args)
