@@ -110,6 +110,7 @@ SIGNATURES = {
     "csx_tune_lookup": (_int, [_u32, _u64, ctypes.c_double, _vp]),
     "csx_compact_auto": (_int, [_vp, _vp, _vp, _u64, _u32, _int, ctypes.c_double, _vp, _vp, _vp, _sz, _vp]),
     "csx_autotune": (_int, [_u32, _u64, ctypes.c_double, _u32, _vp, ctypes.POINTER(ctypes.c_double), _vp]),
+    "csx_autotune_inputs": (_int, [_vp, _vp, _vp, _u64, _u32, _int, _u32, _vp, _sz, _vp, ctypes.POINTER(ctypes.c_double), _vp, _vp]),
     "csx_autotune_batch": (_int, [_vp, _u32, _u64, _u32, _int, _vp, _vp, _sz, _u32, _vp, ctypes.POINTER(ctypes.c_double), _vp]),
     "csx_compact_batch_config": (_int, [_vp, _u32, _u64, _u32, _int, _vp, _vp, _vp, _sz, _vp]),
     "csx_tune_save": (_int, [ctypes.c_char_p]),
@@ -370,6 +371,25 @@ def autotune(elem_size, n, selectivity, reps=10, stream=None):
     ms = ctypes.c_double(0)
     _check(lib().csx_autotune(elem_size, n, float(selectivity), reps, ctypes.byref(c), ctypes.byref(ms), _stream(stream)))
     return c, ms.value
+
+
+def autotune_inputs(inp, mask, n=None, bit_order=MSB_FIRST, reps=10, out=None, workspace=None, stream=None):
+    """Times every candidate on the caller's column and mask and records the
+    fastest under (width, n, k/n); returns (Config, median ms, k).  `out`
+    holds the compacted column afterwards."""
+    _require_device(inp, mask, out)
+    n = inp.numel() if n is None else n
+    w = inp.element_size()
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=inp.dtype, device=inp.device)
+    if workspace is None:
+        workspace = alloc_workspace(auto_workspace_size(n, w), inp.device)
+    c = Config()
+    ms = ctypes.c_double(0)
+    k = _u64(0)
+    _check(lib().csx_autotune_inputs(_ptr(inp), _ptr(mask), _ptr(out), n, w, bit_order, reps, _ptr(workspace), workspace.numel(), ctypes.byref(c),
+                                     ctypes.byref(ms), ctypes.byref(k), _stream(stream)))
+    return c, ms.value, int(k.value)
 
 
 def autotune_batch(table, elem_size, max_total, counts, workspace, reps=5, bit_order=MSB_FIRST, stream=None):

@@ -1260,6 +1260,46 @@ template <typename F> csx_status time_median(F&& fn, uint32_t reps, hipStream_t 
     return CSX_OK;
 }
 
+// Every candidate of the autotuner timed on one column (median of reps after
+// a warm-up, the whole call); all must give the same count, returned in *k.
+// d_count null: the count goes to the workspace header.
+csx_status tune_candidates(const void* in, const uint8_t* mask, void* out, uint64_t n, uint32_t elem_size, int bit_order, uint64_t* d_count, void* ws,
+                           size_t ws_bytes, uint32_t reps, hipStream_t stream, csx_config* best, double* best_t, uint64_t* k)
+{
+    std::vector<csx_config> cand;
+    for (uint32_t chunk : {256u, 1024u, 4096u})
+        for (uint32_t vpt : {2u, 4u, 8u, 16u})
+            if (vpt != 16 || elem_size >= 2) cand.push_back(csx_config{CSX_PIPELINE_THREE_PHASE, chunk, vpt, 0});
+    cand.push_back(csx_config{CSX_PIPELINE_TWO_PASS, 1024, 8, 0});
+    cand.push_back(csx_config{CSX_PIPELINE_SPARSE, 1024, 8, 0});
+    cand.push_back(csx_config{CSX_PIPELINE_SPARSE, 4096, 8, 0});
+    cand.push_back(csx_config{CSX_PIPELINE_FUSED, 1024, 8, 0});
+
+    auto hdr = static_cast<csx::WsHeader*>(ws);
+    const uint64_t* count_src = d_count ? d_count : &hdr->count;
+    uint64_t first_count = 0;
+    for (size_t i = 0; i < cand.size(); i++) {
+        const csx_config c = cand[i];
+        double t = 0.0;
+        csx_status s = time_median(
+            [&] { return compact_impl(in, mask, out, n, elem_size, c.chunk_len, bit_order, c.pipeline, c.vpt, d_count, nullptr, ws, ws_bytes, stream); },
+            reps, stream, &t);
+        if (s) return s;
+        uint64_t kc = 0;
+        if ((s = finish_host_count(&kc, count_src, hdr, stream))) return s;
+        if (i == 0) first_count = kc;
+        if (kc != first_count)
+            return fail(CSX_ERR_DEVICE, "configurations disagree on the selected count (%llu, %llu)", (unsigned long long)first_count,
+                        (unsigned long long)kc);
+        if (i == 0 || t < *best_t) {
+            *best_t = t;
+            *best = c;
+        }
+    }
+    *k = first_count;
+    return CSX_OK;
+}
+
 } // namespace
 } // extern "C++"
 
@@ -1281,40 +1321,33 @@ csx_status csx_autotune(uint32_t elem_size, uint64_t n, double selectivity, uint
     CSX_HIP(buf.alloc(&cnt, sizeof(uint64_t)), "tune alloc");
     if ((s = csx_gen_values(in, n, elem_size, CSX_VALUES_INTEGER, 0x74756e65ull, stream))) return s;
     if ((s = csx_gen_mask_bernoulli(static_cast<uint8_t*>(mask), n, selectivity, 0x6d61736bull, CSX_MSB_FIRST, stream))) return s;
-
-    std::vector<csx_config> cand;
-    for (uint32_t chunk : {256u, 1024u, 4096u})
-        for (uint32_t vpt : {2u, 4u, 8u, 16u})
-            if (vpt != 16 || elem_size >= 2) cand.push_back(csx_config{CSX_PIPELINE_THREE_PHASE, chunk, vpt, 0});
-    cand.push_back(csx_config{CSX_PIPELINE_TWO_PASS, 1024, 8, 0});
-    cand.push_back(csx_config{CSX_PIPELINE_SPARSE, 1024, 8, 0});
-    cand.push_back(csx_config{CSX_PIPELINE_SPARSE, 4096, 8, 0});
-    cand.push_back(csx_config{CSX_PIPELINE_FUSED, 1024, 8, 0});
-
     double best_t = 0.0;
-    uint64_t first_count = 0;
-    for (size_t i = 0; i < cand.size(); i++) {
-        const csx_config c = cand[i];
-        double t = 0.0;
-        s = time_median(
-            [&] {
-                return compact_impl(in, static_cast<const uint8_t*>(mask), out, n, elem_size, c.chunk_len, CSX_MSB_FIRST, c.pipeline, c.vpt,
-                                    static_cast<uint64_t*>(cnt), nullptr, ws, ws_bytes, stream);
-            },
-            reps, stream, &t);
-        if (s) return s;
-        uint64_t k = 0;
-        if ((s = finish_host_count(&k, static_cast<uint64_t*>(cnt), static_cast<csx::WsHeader*>(ws), stream))) return s;
-        if (i == 0) first_count = k;
-        if (k != first_count) return fail(CSX_ERR_DEVICE, "configurations disagree on the selected count (%llu, %llu)", (unsigned long long)first_count,
-                                          (unsigned long long)k);
-        if (i == 0 || t < best_t) {
-            best_t = t;
-            *best = c;
-        }
-    }
+    uint64_t k = 0;
+    if ((s = tune_candidates(in, static_cast<const uint8_t*>(mask), out, n, elem_size, CSX_MSB_FIRST, static_cast<uint64_t*>(cnt), ws, ws_bytes, reps,
+                             stream, best, &best_t, &k)))
+        return s;
     record_cell(elem_size, n, selectivity, *best, best_t);
     if (best_ms) *best_ms = best_t;
+    return ok();
+}
+
+csx_status csx_autotune_inputs(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, int bit_order, uint32_t reps,
+                               void* d_ws, size_t ws_bytes, csx_config* best, double* best_ms, uint64_t* h_count, hipStream_t stream)
+{
+    csx_status s;
+    if ((s = check_width(elem_size)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
+    if (n == 0 || reps == 0) return fail(CSX_ERR_INVALID, "nothing to time (n = 0 or reps = 0)");
+    if (!best) return fail(CSX_ERR_INVALID, "null result pointer");
+    if (!d_in || !d_mask || !d_out) return fail(CSX_ERR_INVALID, "null input, mask or output pointer");
+    size_t need = 0;
+    if ((s = csx_auto_workspace_size(n, elem_size, &need)) || (s = check_ws(d_ws, ws_bytes, need))) return s;
+    // the count goes to the workspace header (no d_count)
+    double best_t = 0.0;
+    uint64_t k = 0;
+    if ((s = tune_candidates(d_in, d_mask, d_out, n, elem_size, bit_order, nullptr, d_ws, ws_bytes, reps, stream, best, &best_t, &k))) return s;
+    record_cell(elem_size, n, (double)k / (double)n, *best, best_t);
+    if (best_ms) *best_ms = best_t;
+    if (h_count) *h_count = k;
     return ok();
 }
 

@@ -115,7 +115,10 @@ def shapes_sweep(args):
     three-phase call (chunk 1024) with tiles of 2, 4, 8 and 16 vectors per
     thread, and the two-pass call.  Each time the median of `reps` HIP-event
     timings of the whole call; every count checked against the default
-    call's.  Per row also the mask's occupied-line ratio (line_ratio)."""
+    call's.  Per row also the mask's occupied-line ratio (line_ratio).  Then,
+    for 4- and 8-byte elements, the tuned call on Bernoulli 0.5, runs and
+    Zipf masks with the shipped table, and again after csx_autotune_inputs on
+    that column and mask."""
     n = 1 << args.log2n
     masks = {f"bernoulli_{p}": csx.gen_mask_bernoulli(n, p, seed=11) for p in (0.1, 0.25, 0.5, 0.75, 0.9)}
     masks.update({f"runs_{r}": csx.gen_mask_runs(n, r, seed=7) for r in (16, 64, 256, 1024, 4096)})
@@ -150,7 +153,41 @@ def shapes_sweep(args):
                   + f";{row['best']}", flush=True)
         del data, out, ws
         torch.cuda.empty_cache()
-    return rows
+
+    # the tuned call (selectivity k/n) with the shipped table, then after
+    # csx_autotune_inputs on the same column and mask
+    tuned = []
+    print("width;mask;table_config;table_us;column_tuned_config;column_tuned_us", flush=True)
+    for w in (4, 8):
+        data = csx.gen_values(n, w, seed=3)
+        out = torch.empty_like(data)
+        ws = csx.alloc_workspace(csx.auto_workspace_size(n, w))
+        for mname in ("bernoulli_0.5", "runs_256", "runs_1024", "zipf_blocks"):
+            mask = masks[mname]
+            csx.tune_clear()
+            csx.tune_load(csx.TUNED_TABLE)
+            _, k = csx.compact(data, mask, n=n, out=out, workspace=ws)
+            sel = k / n
+
+            def auto():
+                csx.compact_auto(data, mask, sel, n=n, out=out, count=count, workspace=ws, host_count=False)
+
+            before_cfg = csx.tune_lookup(w, n, sel).as_dict()
+            before = median_ms(auto, args.reps)
+            cfg, _, kk = csx.autotune_inputs(data, mask, n=n, reps=5, out=out, workspace=ws)
+            after = median_ms(auto, args.reps)
+            if kk != k or int(count.item()) != k:
+                print(f"count mismatch {w} {mname} (tuned on the column)", file=sys.stderr)
+                sys.exit(1)
+            row = {"width": w, "mask": mname, "table_config": before_cfg, "table_us": round(before * 1000, 1), "column_tuned_config": cfg.as_dict(),
+                   "column_tuned_us": round(after * 1000, 1)}
+            tuned.append(row)
+            print(f"{w};{mname};{before_cfg};{row['table_us']};{cfg.as_dict()};{row['column_tuned_us']}", flush=True)
+        del data, out, ws
+        torch.cuda.empty_cache()
+    csx.tune_clear()
+    csx.tune_load(csx.TUNED_TABLE)
+    return {"tiles": rows, "tuned_on_column": tuned}
 
 
 def main():

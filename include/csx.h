@@ -307,6 +307,19 @@ csx_status csx_compact_auto(const void* d_in, const uint8_t* d_mask, void* d_out
  * records the fastest in the table and returns it.  Allocates and frees its
  * own device buffers and synchronises the stream: an offline step. */
 csx_status csx_autotune(uint32_t elem_size, uint64_t n, double selectivity, uint32_t reps, csx_config* best, double* best_ms, hipStream_t stream);
+/* The same candidates timed on the caller's own column: d_in / d_mask (n
+ * elements, bit_order) into d_out (n elements), with the caller's workspace
+ * of csx_auto_workspace_size bytes.  The fastest is recorded under
+ * (elem_size, n, k/n), k the selected count (returned in *h_count when not
+ * null), and csx_compact_auto uses it from then on for that cell.  The table
+ * is measured on Bernoulli masks; on masks whose selected elements come in
+ * runs (sorted or correlated columns) a larger tile can be faster at the
+ * same selectivity, and tuning on such a column records it.  Runs the call
+ * reps+1 times per candidate (d_out and the workspace are rewritten each
+ * time) and synchronises the stream: an offline step.  No counterpart in the
+ * reference. */
+csx_status csx_autotune_inputs(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, int bit_order, uint32_t reps,
+                               void* d_ws, size_t ws_bytes, csx_config* best, double* best_ms, uint64_t* h_count, hipStream_t stream);
 /* The batched form tuned on the caller's own table (vpt {2, 4, 8} x
  * count_blocks {1024, 2048, 4096, 8192}); the result is recorded for
  * elem_size and used by csx_compact_batch from then on.  Runs the batch

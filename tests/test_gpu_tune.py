@@ -67,6 +67,29 @@ def test_autotune_then_auto(O):
     assert kk == k and (out[:k].cpu().numpy().view(np.uint32) == expect).all()
 
 
+@pytest.mark.parametrize("w", [1, 4, 8])
+def test_autotune_on_callers_column(O, w):
+    """Tuning on the caller's own column and mask (runs of mean 256, LSB-first
+    bits, ragged length): the count and the output left in `out` are the
+    oracle's, the fastest candidate is recorded under (w, n, k/n), and the
+    tuned call then uses it and stays bit-exact."""
+    n = 300007
+    data = O.fill_values(n, NP[w], seed=77 + w)
+    mask = O.mask_geometric_runs(n, 256, seed=78 + w, bit_order=1)
+    expect, k = O.compact(data, mask, n, bit_order=1)
+    din = torch.from_numpy(data.view(np.uint8)).cuda().view(VIEW[w])
+    m = torch.from_numpy(mask).cuda()
+    cfg, ms, kk = csx.autotune_inputs(din, m, n=n, bit_order=1, reps=3)
+    assert kk == k and ms > 0
+    got = csx.tune_lookup(w, n, k / n)
+    assert got.as_dict() == cfg.as_dict()
+    out, kk = csx.compact_auto(din, m, k / n, n=n, bit_order=1)
+    assert kk == k and (out[:k].cpu().numpy().view(NP[w]) == expect).all()
+    out_after_tuning = torch.empty_like(din)
+    csx.autotune_inputs(din, m, n=n, bit_order=1, reps=1, out=out_after_tuning)
+    assert (out_after_tuning[:k].cpu().numpy().view(NP[w]) == expect).all()
+
+
 def test_batch_configs_and_autotune(O):
     rng = np.random.default_rng(21)
     lens = [int(x) for x in np.exp(rng.uniform(np.log(1), np.log(200000), 120))] + [0, 1, 31]

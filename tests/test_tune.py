@@ -80,6 +80,12 @@ def test_config_checks_before_device_work():
     ms = ctypes.c_double()
     assert L.csx_autotune(4, 0, 0.5, 3, ctypes.byref(c), ctypes.byref(ms), None) == 1
     assert L.csx_autotune(4, 100, 1.5, 3, ctypes.byref(c), ctypes.byref(ms), None) == 1
+    fake = ctypes.c_void_p(1 << 20)
+    # tuning on the caller's column: null buffers, zero reps, bad width, short workspace
+    assert L.csx_autotune_inputs(None, None, None, 100, 4, 0, 3, fake, 1 << 20, ctypes.byref(c), ctypes.byref(ms), None, None) == 1
+    assert L.csx_autotune_inputs(fake, fake, fake, 100, 4, 0, 0, fake, 1 << 20, ctypes.byref(c), ctypes.byref(ms), None, None) == 1
+    assert L.csx_autotune_inputs(fake, fake, fake, 100, 3, 0, 3, fake, 1 << 20, ctypes.byref(c), ctypes.byref(ms), None, None) == 1
+    assert L.csx_autotune_inputs(fake, fake, fake, 1 << 20, 4, 0, 3, fake, 64, ctypes.byref(c), ctypes.byref(ms), None, None) == 2
     assert csx.auto_workspace_size(1 << 20, 4) == csx.workspace_size(1 << 20, 4, 256)
 
 
