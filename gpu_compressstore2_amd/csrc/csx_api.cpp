@@ -159,6 +159,16 @@ csx_status check_ws(const void* ws, size_t have, size_t need)
     return CSX_OK;
 }
 
+// Whether the byte ranges [a, a + na) and [b, b + nb) share a byte.  The
+// kernels never compact in place: a tile's output can land on input or mask
+// bytes that another tile, running at the same time, has not read yet.
+bool overlaps(const void* a, uint64_t na, const void* b, uint64_t nb)
+{
+    if (!a || !b || na == 0 || nb == 0) return false;
+    const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+    return x >= y ? x - y < nb : y - x < na;
+}
+
 hipError_t dispatch_tile(uint32_t w, int mode, bool staged, int order, bool aligned, const csx::TileArgs& a, hipStream_t s, int vpt = 8)
 {
     switch (w) {
@@ -467,6 +477,8 @@ csx_status csx_writeout_ex(const csx_writeout_args* wa, hipStream_t stream)
     if (variant < CSX_WRITEOUT_STAGED || variant > CSX_WRITEOUT_SPARSE) return fail(CSX_ERR_INVALID, "unknown writeout variant %d", variant);
     if (n == 0) return ok();
     if (!wa->in || !wa->mask || !wa->offsets || !wa->out) return fail(CSX_ERR_INVALID, "null input, mask, offsets or output pointer");
+    if (overlaps(wa->out, n * elem_size, wa->in, n * elem_size) || overlaps(wa->out, n * elem_size, wa->mask, (n + 7) / 8))
+        return fail(CSX_ERR_INVALID, "the output overlaps the input or the mask (the writeout is not in place)");
     if (((uintptr_t)wa->in % elem_size) || ((uintptr_t)wa->out % elem_size))
         return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
     csx::TileArgs a{};
@@ -567,6 +579,8 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
     const CompactPlan p = plan_compact(n, elem_size, chunk_len);
     if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
     if (n > 0 && (!d_in || !d_mask || !d_out)) return fail(CSX_ERR_INVALID, "null input, mask or output pointer");
+    if (overlaps(d_out, n * elem_size, d_in, n * elem_size) || overlaps(d_out, n * elem_size, d_mask, (n + 7) / 8))
+        return fail(CSX_ERR_INVALID, "the output overlaps the input or the mask (compaction is not in place)");
     if (((uintptr_t)d_in % elem_size) || ((uintptr_t)d_out % elem_size))
         return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
     // the writeout's grid is one workgroup per tile of the chosen size
@@ -723,6 +737,8 @@ csx_status csx_pattern_compact(const void* d_in, void* d_out, uint64_t n, uint32
     uint64_t k = 0;
     if ((s = check_width(elem_size)) || (s = check_n(n)) || (s = csx_pattern_count(n, pattern, pattern_length, &k))) return s;
     if (k > 0 && (!d_in || !d_out)) return fail(CSX_ERR_INVALID, "null input or output pointer");
+    if (k > 0 && overlaps(d_out, n * elem_size, d_in, n * elem_size))
+        return fail(CSX_ERR_INVALID, "the output overlaps the input (compaction is not in place)");
     if (((uintptr_t)d_in % elem_size) || ((uintptr_t)d_out % elem_size))
         return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
     // tile size, measured at 2^29 on 32-bit patterns of 12.5 % and 62.5 %
@@ -814,6 +830,8 @@ csx_status csx_compact_streams(const void* d_in, const uint8_t* d_mask, void* d_
     const StreamsPlan p = plan_streams(n, elem_size, chunk_len, stream_count);
     if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
     if (n > 0 && (!d_in || !d_mask || !d_out)) return fail(CSX_ERR_INVALID, "null input, mask or output pointer");
+    if (overlaps(d_out, n * elem_size, d_in, n * elem_size) || overlaps(d_out, n * elem_size, d_mask, (n + 7) / 8))
+        return fail(CSX_ERR_INVALID, "the output overlaps the input or the mask (compaction is not in place)");
     if (((uintptr_t)d_in % elem_size) || ((uintptr_t)d_out % elem_size))
         return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
     auto base = static_cast<uint8_t*>(d_ws);
@@ -1116,6 +1134,7 @@ csx_status csx_predicate_mask(const void* d_in, uint64_t n, const csx_predicate*
     uint32_t w = 0;
     if ((s = check_pred(pred, &p, &w)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
     if (n > 0 && (!d_in || !d_mask)) return fail(CSX_ERR_INVALID, "null input or mask pointer");
+    if (n > 0 && overlaps(d_mask, (n + 7) / 8, d_in, n * w)) return fail(CSX_ERR_INVALID, "the mask overlaps the column");
     if ((uintptr_t)d_in % w) return fail(CSX_ERR_INVALID, "input must be aligned to the element size");
     if (d_count) CSX_HIP(csx::launch_store_u64(d_count, 0, stream), "csx_predicate_mask count");
     if (n == 0) return ok();
@@ -1146,6 +1165,8 @@ csx_status csx_compact_predicate(const void* d_in, uint64_t n, const csx_predica
     if ((s = check_pred(pred, &p, &w)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
     if ((s = check_ws(d_ws, ws_bytes, pred_ws_total(n, w)))) return s;
     if (n > 0 && (!d_in || !d_out)) return fail(CSX_ERR_INVALID, "null input or output pointer");
+    if (n > 0 && (overlaps(d_out, n * w, d_in, n * w) || overlaps(d_mask, (n + 7) / 8, d_in, n * w) || overlaps(d_mask, (n + 7) / 8, d_out, n * w)))
+        return fail(CSX_ERR_INVALID, "the column, the output and the mask must not overlap (compaction is not in place)");
     if (((uintptr_t)d_in % w) || ((uintptr_t)d_out % w)) return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
     const CompactPlan cp = plan_compact(n, w, kPredChunk);
     if (!launch_fits(cp.fused_tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)cp.fused_tiles);

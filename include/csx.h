@@ -165,7 +165,9 @@ csx_status csx_writeout_w8(const uint64_t* d_in, const uint8_t* d_mask, const ui
  * Full compaction: d_out[0..k) = selected elements of d_in[0..n) in order,
  * k written to *d_count (device, nullable) and, if h_count is not null, to
  * *h_count after synchronising the stream.  d_out must hold k elements
- * (n is always enough); nothing at or past d_out[k] is written.
+ * (n is always enough); nothing at or past d_out[k] is written.  Not in
+ * place: d_out's n elements must not overlap d_in or d_mask (checked, as
+ * for every entry point below that takes both; CSX_ERR_INVALID).
  * chunk_len only matters for CSX_PIPELINE_THREE_PHASE and CSX_PIPELINE_SPARSE.
  * CSX_PIPELINE_THREE_PHASE is the default of the language bindings.
  * Replaces the benchN compositions (src/benchmarks.cuh:153-440) -- bench10
@@ -206,7 +208,8 @@ csx_status csx_compact_streams(const void* d_in, const uint8_t* d_mask, void* d_
  * workgroup per tile of the bound, and those past the real tiles only exit);
  * a table exceeding it sets the workspace error word (csx_workspace_error)
  * and leaves the excess uncompacted.  A workspace sized for a larger bound
- * serves any smaller one.  No
+ * serves any smaller one.  A column's out must not overlap any column's in
+ * or mask (the table is in device memory, so this is not checked).  No
  * counterpart in the reference beyond its two-stream split
  * (src/benchmarks.cuh:195-293); MSB- or LSB-first masks, 1/2/4/8-byte
  * elements, all columns of one width. */

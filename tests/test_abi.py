@@ -103,6 +103,32 @@ def test_unknown_pipeline_rejected():
     assert b"unknown pipeline" in L.csx_last_error()
 
 
+def test_in_place_calls_rejected_before_device_work():
+    """Compaction is not in place: an output range that shares a byte with
+    the input or the mask is refused with CSX_ERR_INVALID before any launch
+    (addresses only, nothing is dereferenced)."""
+    L = csx.lib()
+    n, w = 1000, 4
+    base = 1 << 30
+    data, mask, ws = base, base + (1 << 24), base + (1 << 26)
+    V = ctypes.c_void_p
+    for out in (data, data + 4 * (n - 1), data - 4 * (n - 1), mask - 4, mask + 100):
+        st = L.csx_compact(V(data), V(mask), V(out), n, w, 1024, 0, csx.PIPELINE_THREE_PHASE, None, None, V(ws), 1 << 30, None)
+        assert st == 1 and b"overlaps" in L.csx_last_error(), hex(out - base)
+        wa = csx.WriteoutArgs(V(data), V(mask), V(out), V(ws), 0, None, None, n, w, 1024, 0, csx.WRITEOUT_STAGED)
+        assert L.csx_writeout_ex(ctypes.byref(wa), None) == 1 and b"overlaps" in L.csx_last_error()
+    st = L.csx_pattern_compact(V(data), V(data + 8), n, w, 0xAAAAAAAA, 32, None, None, None)
+    assert st == 1 and b"overlaps" in L.csx_last_error()
+    u32 = csx.TYPE_NAMES.index("uint32")
+    q = csx.predicate(u32, csx.PRED_GT, 5)
+    need = csx.predicate_workspace_size(n, u32)
+    for out, m in ((data + 64, None), (data + (1 << 20), V(data + 100)), (data + (1 << 20), V(data + (1 << 20) + 8))):
+        st = L.csx_compact_predicate(V(data), n, ctypes.byref(q), V(out), m, 0, None, None, V(ws), need, None)
+        assert st == 1 and b"must not overlap" in L.csx_last_error()
+    assert L.csx_predicate_mask(V(data), n, ctypes.byref(q), V(data + 12), 0, None, None) == 1
+    assert b"overlaps" in L.csx_last_error()
+
+
 def test_writeout_variant_range():
     L = csx.lib()
     # the sparse variant is known: rejected for the null pointers, not the variant

@@ -95,6 +95,7 @@ def test_tile_grid_bound_checked_for_the_chosen_tile():
     any device work."""
     L = csx.lib()
     c = csx.Config(csx.PIPELINE_THREE_PHASE, 1 << 17, 2, 0)
-    fake = ctypes.c_void_p(1 << 20)
-    st = L.csx_compact_config(fake, fake, fake, 1 << 38, 1, 0, ctypes.byref(c), None, None, fake, 2**63, None)
+    # disjoint fake ranges (2^38 input bytes, 2^35 mask bytes, 2^38 output bytes), never dereferenced
+    fin, fmask, fout, fws = (ctypes.c_void_p(a) for a in (1 << 40, 1 << 41, 1 << 42, 1 << 20))
+    st = L.csx_compact_config(fin, fmask, fout, 1 << 38, 1, 0, ctypes.byref(c), None, None, fws, 2**63, None)
     assert st == 4 and "exceed one launch" in L.csx_last_error().decode()
