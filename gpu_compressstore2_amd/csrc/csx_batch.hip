@@ -7,7 +7,8 @@
 //   K0 plan    one workgroup: tile prefix over the segments, the total tile
 //              count, zeroed per-segment counts and look-back state
 //   K1 counts  persistent grid walking the (segment, tile) queue, one wave
-//              per tile: selected count of every tile
+//              load per tile (per group of tiles when a tile has fewer than
+//              64 mask vectors): selected count of every tile
 //   K2 scan    the look-back scan over all tile counts (length from device)
 //   K3 write   one workgroup per (segment, tile): the tile engine with the
 //              tile's offset made local to its segment; the last tile of a
@@ -90,11 +91,101 @@ __device__ __forceinline__ uint32_t find_segment(const uint64_t* tile_prefix, ui
 // segments included), and kCountUnroll tiles' mask loads are in flight together.
 constexpr int kCountUnroll = 4;
 
+// Tiles of fewer than 64 mask vectors (VEC = TE / 128): one wave load covers
+// TPW = 64 / VEC consecutive tiles, lane L on vector L % VEC of the load's
+// tile L / VEC, and each group of VEC lanes reduces its own tile's count.  The
+// wave's segment cursor follows each load's first tile; a lane whose tile lies
+// past that segment's end finds its own segment (rare: a column end inside a
+// load).
+template <int W, int ORDER, uint32_t VEC, uint64_t TE>
+__device__ __forceinline__ void batch_counts_packed(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, uint32_t* tcounts)
+{
+    constexpr uint32_t TPW = kWave / VEC;
+    static_assert(VEC * TPW == kWave && TE == 128ull * VEC, "tiles pack a wave load exactly");
+    const uint32_t lane = lane_id();
+    const uint32_t j = lane / VEC, v = lane % VEC;
+    const uint64_t total = tile_prefix[nseg];
+    const uint64_t wave = (blockIdx.x * 256ull + threadIdx.x) / kWave;
+    const uint64_t waves = gridDim.x * (256ull / kWave);
+    const uint64_t per = (total + waves - 1) / waves;
+    const uint64_t g0 = wave * per;
+    const uint64_t g1 = g0 + per < total ? g0 + per : total;
+    if (g0 >= g1) return;  // wave-uniform
+    uint32_t s = find_segment(tile_prefix, nseg, g0);
+    uint64_t s_first = tile_prefix[s], s_end = tile_prefix[s + 1];
+    Segment seg = segs[s];
+    for (uint64_t g = g0; g < g1; g += (uint64_t)kCountUnroll * TPW) {
+        u32x4 x[kCountUnroll];
+        uint64_t n_of[kCountUnroll], e_of[kCountUnroll];
+        bool live[kCountUnroll];
+#pragma unroll
+        for (int u = 0; u < kCountUnroll; u++) {
+            const uint64_t gb = g + (uint64_t)u * TPW;  // the load's first tile (wave-uniform)
+            const uint64_t gu = gb + j;                  // this lane's tile
+            x[u] = u32x4{0, 0, 0, 0};
+            n_of[u] = 0;
+            e_of[u] = 0;
+            live[u] = gu < g1;
+            if (gb >= g1) continue;
+            while (gb >= s_end) {  // next non-empty segment
+                s++;
+                s_first = s_end;
+                s_end = tile_prefix[s + 1];
+                seg = segs[s];
+            }
+            uint64_t lf = s_first, le = s_end;
+            const uint8_t* lmask = seg.mask;
+            uint64_t ln = seg.n;
+            if (live[u] && gu >= le) {
+                // gu < total = tile_prefix[nseg]: the walk stops before nseg
+                uint32_t ls = s;
+                while (gu >= le) {
+                    ls++;
+                    lf = le;
+                    le = tile_prefix[ls + 1];
+                }
+                lmask = segs[ls].mask;
+                ln = segs[ls].n;
+            }
+            if (live[u]) {
+                const uint64_t lt = gu - lf;
+                const uint64_t nbytes = (ln + 7) / 8;
+                const uint64_t vec = lt * VEC + v;
+                x[u] = ((uintptr_t)lmask & 15) == 0 ? load_mask16<true>(lmask, vec * 16, nbytes) : load_mask16<false>(lmask, vec * 16, nbytes);
+                n_of[u] = ln;
+                e_of[u] = lt * TE;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kCountUnroll; u++) {
+            const uint64_t gb = g + (uint64_t)u * TPW;
+            if (gb >= g1) break;  // wave-uniform
+            uint32_t w[4] = {to_lsb_word<ORDER>(x[u].x), to_lsb_word<ORDER>(x[u].y), to_lsb_word<ORDER>(x[u].z), to_lsb_word<ORDER>(x[u].w)};
+            const uint64_t e0 = e_of[u] + 128ull * v;
+            if (live[u] && e0 + 128 > n_of[u]) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint64_t b = e0 + 32 * k;
+                    w[k] = b >= n_of[u] ? 0u : low_bits(w[k], (uint32_t)(n_of[u] - b));
+                }
+            }
+            uint32_t c = live[u] ? popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]) : 0u;
+#pragma unroll
+            for (uint32_t d = VEC / 2; d >= 1; d >>= 1) c += __shfl_xor(c, d, kWave);
+            if (live[u] && v == 0) tcounts[gb + j] = c;
+        }
+    }
+}
+
 template <int W, int ORDER, int VPT>
 __global__ __launch_bounds__(256) void k_batch_counts(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, uint32_t* tcounts)
 {
     constexpr uint64_t TE = TileCfg<W, VPT>::TE;
     constexpr uint32_t VEC_PER_TILE = (uint32_t)(TE / 128);
+    if constexpr (VEC_PER_TILE < kWave) {
+        batch_counts_packed<W, ORDER, VEC_PER_TILE, TE>(segs, nseg, tile_prefix, tcounts);
+        return;
+    }
     constexpr uint32_t VPL = (VEC_PER_TILE + kWave - 1) / kWave;  // mask vectors per lane per tile
     const uint32_t lane = lane_id();
     const uint64_t total = tile_prefix[nseg];
