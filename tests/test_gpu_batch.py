@@ -55,6 +55,31 @@ def test_batch_parity(O, w, order):
     check(cols, outs, counts, w)
 
 
+@pytest.mark.parametrize("w", [1, 2, 4, 8])
+def test_batch_every_tile_size_and_count_grid(O, w):
+    """Every tile size of the batched form (2, 4 and 8 vectors per thread: for
+    2- to 8-byte elements 2 to 8 tiles share one wave load in the count
+    kernel) and count grids from one workgroup up, on columns shorter and
+    longer than a tile, with empty ones between."""
+    T = TILE[w] // 4  # the 2-vector tile
+    lengths = [0, 1, T - 1, T, T + 1, 0, 0, 5, 2 * T + 3, 17, 9 * T + 1, 0, 3, 4 * T, 100003]
+    for order in (oracle.MSB_FIRST, oracle.LSB_FIRST):
+        cols, ins, masks, outs = build(O, w, lengths, seed=40 + w + order, order=order)
+        table = csx.segment_table(ins, masks, outs, lengths)
+        total = sum(lengths)
+        ws = csx.alloc_workspace(csx.batch_workspace_size(len(lengths), total, w))
+        counts = torch.zeros(len(lengths), dtype=torch.int64, device="cuda")
+        for vpt in (2, 4, 8):
+            for cb in (1, 7, 64, 4096):
+                print(f"batch w={w} order={order} vpt={vpt} count_blocks={cb}", flush=True)
+                for o in outs:
+                    o.fill_(-1)
+                csx.compact_batch_config(table, w, total, csx.Config(csx.PIPELINE_THREE_PHASE, 0, vpt, cb), counts=counts, workspace=ws,
+                                         bit_order=order)
+                torch.cuda.synchronize()
+                check(cols, outs, counts, w)
+
+
 def test_batch_many_log_uniform_columns(O):
     rng = np.random.default_rng(4)
     lengths = [int(x) for x in np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 16), 300))]
