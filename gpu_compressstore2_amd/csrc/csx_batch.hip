@@ -263,14 +263,25 @@ __global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32
     using C = TileCfg<W, VPT>;
     __shared__ TileShared<C> sh;
     __shared__ uint32_t s_seg;
+    __shared__ uint32_t s_live;
     // runs of consecutive tiles per XCD (see xcd_runs)
     const uint64_t g = xcd_runs(blockIdx.x, gridDim.x);
-    if (g >= tile_prefix[nseg]) return;  // surplus of the host's bound: uniform exit
+    // The real tile count, the segment search and the tile's global offset are
+    // requested together (the search reads only prefixes below nseg, and g is
+    // below the grid, the max_tiles words of goff), so the tile waits one
+    // round trip less before its mask load; a surplus tile of the host's
+    // bound leaves after the barrier.
     if (threadIdx.x < kWave) {
+        const uint64_t total = tile_prefix[nseg];
         const uint32_t s0 = find_segment(tile_prefix, nseg, g);  // wave 0 together
-        if (threadIdx.x == 0) s_seg = s0;
+        if (threadIdx.x == 0) {
+            s_seg = s0;
+            s_live = g < total;
+        }
     }
+    const uint64_t gofs = goff[g];
     __syncthreads();
+    if (!s_live) return;  // uniform
     const uint32_t s = s_seg;
     const Segment seg = segs[s];
     const uint64_t first = tile_prefix[s];
@@ -279,7 +290,7 @@ __global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32
     a.mask = seg.mask;
     a.out = static_cast<uint8_t*>(seg.out);
     a.n = seg.n;
-    a.given_off = goff[g] - goff[first];
+    a.given_off = gofs - goff[first];
     a.count_out = counts + s;
     if ((((uintptr_t)a.in) | ((uintptr_t)a.mask)) & 15)
         tile_body<W, ORDER, false, kModeGiven, true, VPT>(a, sh, g - first);
