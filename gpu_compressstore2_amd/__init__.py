@@ -106,6 +106,7 @@ SIGNATURES = {
     "csx_profile_begin": (_int, []),
     "csx_profile_end": (_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
     "csx_compact_config": (_int, [_vp, _vp, _vp, _u64, _u32, _int, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "csx_compact_noalloc": (_int, [_vp, _vp, _vp, _u64, _u32, _int, _vp, _vp, ctypes.POINTER(_u64), _vp]),
     "csx_auto_workspace_size": (_int, [_u64, _u32, ctypes.POINTER(_sz)]),
     "csx_tune_lookup": (_int, [_u32, _u64, ctypes.c_double, _vp]),
     "csx_compact_auto": (_int, [_vp, _vp, _vp, _u64, _u32, _int, ctypes.c_double, _vp, _vp, _vp, _sz, _vp]),
@@ -339,6 +340,25 @@ def compact_config(inp, mask, cfg, n=None, bit_order=MSB_FIRST, out=None, count=
     h = _u64(0)
     _check(lib().csx_compact_config(_ptr(inp), _ptr(mask), _ptr(out), n, w, bit_order, ctypes.byref(cfg), _ptr(count),
                                     ctypes.byref(h) if host_count else None, _ptr(workspace), workspace.numel(), _stream(stream)))
+    return out, (int(h.value) if host_count else count)
+
+
+NOALLOC_SCRATCH_BYTES = 32 << 20
+
+
+def compact_noalloc(inp, mask, cfg, n=None, bit_order=MSB_FIRST, out=None, count=None, stream=None, host_count=True):
+    """csx_compact_noalloc: csx_compact_config in the library's static device
+    scratch (no workspace, no allocation by the call); returns (out, k)."""
+    _require_device(inp, mask, out, count)
+    n = inp.numel() if n is None else n
+    w = inp.element_size()
+    if out is None:
+        out = torch.empty(max(n, 1), dtype=inp.dtype, device=inp.device)
+    if count is None:
+        count = torch.empty(1, dtype=torch.int64, device=inp.device)
+    h = _u64(0)
+    _check(lib().csx_compact_noalloc(_ptr(inp), _ptr(mask), _ptr(out), n, w, bit_order, ctypes.byref(cfg), _ptr(count),
+                                     ctypes.byref(h) if host_count else None, _stream(stream)))
     return out, (int(h.value) if host_count else count)
 
 

@@ -46,7 +46,7 @@ def _units():
     ]
     for w in (1, 2, 4, 8):
         units.append(("csx_tile.hip", f"csx_tile_w{w}.o", DEVICE + [f"-DCSX_TILE_W={w}"]))
-    for extra in ("csx_gen.hip", "csx_batch.hip", "csx_predicate.hip"):
+    for extra in ("csx_gen.hip", "csx_batch.hip", "csx_predicate.hip", "csx_noalloc.hip"):
         if os.path.exists(os.path.join(CSRC, extra)):
             units.append((extra, extra.replace(".hip", ".o"), DEVICE))
     return units

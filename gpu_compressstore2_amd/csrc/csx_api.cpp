@@ -568,8 +568,12 @@ static csx_status check_vpt(uint32_t vpt, uint32_t elem_size)
 
 static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, uint32_t chunk_len,
                                int bit_order, int pipeline, uint32_t vpt, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes,
-                               hipStream_t stream)
+                               hipStream_t stream, bool profile = true)
 {
+    // the dominant kernel between csx_profile_begin's event pairs (creating
+    // the events is the only resource this path acquires; `profile` = false
+    // keeps it out)
+    auto timed = [&](hipStream_t st, auto&& fn) -> hipError_t { return profile ? ::timed(st, fn) : fn(); };
     csx_status s;
     if ((s = check_width(elem_size)) || (s = check_chunk(chunk_len)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
     if (pipeline != CSX_PIPELINE_FUSED && pipeline != CSX_PIPELINE_THREE_PHASE && pipeline != CSX_PIPELINE_TWO_PASS && pipeline != CSX_PIPELINE_SPARSE)
@@ -683,6 +687,20 @@ csx_status csx_compact_config(const void* d_in, const uint8_t* d_mask, void* d_o
     return compact_impl(d_in, d_mask, d_out, n, elem_size, cfg->chunk_len, bit_order, cfg->pipeline, cfg->vpt, d_count, h_count, d_ws, ws_bytes,
                         stream);
 }
+
+} // extern "C"
+
+namespace csx {
+csx_status compact_in_workspace(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, int bit_order,
+                                const csx_config* cfg, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream)
+{
+    if (!cfg) return fail(CSX_ERR_INVALID, "null configuration");
+    return compact_impl(d_in, d_mask, d_out, n, elem_size, cfg->chunk_len, bit_order, cfg->pipeline, cfg->vpt, d_count, h_count, d_ws, ws_bytes,
+                        stream, false);
+}
+} // namespace csx
+
+extern "C" {
 
 #define CSX_COMPACT_W(W, T)                                                                                                                   \
     csx_status csx_compact_w##W(const T* d_in, const uint8_t* d_mask, T* d_out, uint64_t n, uint32_t chunk_len, int bit_order, int pipeline, \

@@ -12,6 +12,10 @@ namespace csx {
 // the calling thread's csx_last_error() message (csx_api.cpp)
 csx_status set_error(csx_status s, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 csx_status clear_error();
+// csx_compact_config's body with no profiling event pairs (csx_api.cpp); the
+// no-allocation entry point runs it in its static scratch
+csx_status compact_in_workspace(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, int bit_order,
+                                const csx_config* cfg, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream);
 
 // With `hdr` set, the popcount kernel also initialises the look-back state of
 // `init_tiles` scan tiles (saves the init launch before launch_scan).

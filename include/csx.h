@@ -332,6 +332,20 @@ csx_status csx_autotune_batch(const csx_segment* d_segments, uint32_t nseg, uint
                               hipStream_t stream);
 csx_status csx_compact_batch_config(const csx_segment* d_segments, uint32_t nseg, uint64_t max_total_elements, uint32_t elem_size, int bit_order,
                                     const csx_config* cfg, uint64_t* d_counts, void* d_ws, size_t ws_bytes, hipStream_t stream);
+/* csx_compact_config with no workspace argument.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * scratch is a device array inside the library, CSX_NOALLOC_SCRATCH_BYTES on
+ * each GPU, whose address is looked up on the first call on that device; it
+ * is held until the process ends.  The call allocates nothing.  A request
+ * whose workspace (csx_workspace_size(n, elem_size, cfg->chunk_len)) exceeds
+ * the scratch returns CSX_ERR_UNSUPPORTED before any device work: 32 MiB
+ * covers about 2^31 four-byte elements at chunk 1024.  Every call on one GPU
+ * shares that GPU's scratch, so calls that overlap on one GPU (other streams
+ * or host threads) race: the caller serialises them. */
+#define CSX_NOALLOC_SCRATCH_BYTES (UINT64_C(32) << 20)
+csx_status csx_compact_noalloc(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, int bit_order,
+                               const csx_config* cfg, uint64_t* d_count, uint64_t* h_count, hipStream_t stream);
 /* the table as text (one measured cell a line); load adds to the table */
 csx_status csx_tune_save(const char* path);
 csx_status csx_tune_load(const char* path);

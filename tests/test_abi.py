@@ -84,6 +84,21 @@ def test_workspace_checks():
     assert L.csx_compact(None, None, None, 1 << 20, 4, 1024, 0, 0, None, None, ctypes.c_void_p(4097), need, None) == 2
 
 
+def test_noalloc_rejects_oversized_request_before_device_work():
+    """csx_compact_noalloc checks the request against its static scratch
+    before it looks the scratch up on the device (fake pointers, never
+    dereferenced)."""
+    L = csx.lib()
+    assert csx.workspace_size(1 << 30, 4, 32) > csx.NOALLOC_SCRATCH_BYTES
+    cfg = csx.Config(csx.PIPELINE_THREE_PHASE, 32, 8, 0)
+    fake = ctypes.c_void_p(4096)
+    assert L.csx_compact_noalloc(fake, fake, ctypes.c_void_p(1 << 40), 1 << 30, 4, 0, ctypes.byref(cfg), None, None, None) == 4
+    assert b"static scratch" in L.csx_last_error()
+    assert L.csx_compact_noalloc(fake, fake, fake, 100, 4, 0, None, None, None, None) == 1
+    bad = csx.Config(csx.PIPELINE_THREE_PHASE, 100, 8, 0)
+    assert L.csx_compact_noalloc(fake, fake, fake, 100, 4, 0, ctypes.byref(bad), None, None, None) == 1
+
+
 def test_null_data_pointers_rejected():
     L = csx.lib()
     need = csx.workspace_size(100, 4, 1024)
