@@ -124,12 +124,16 @@ ScanPlan plan_scan(uint64_t chunks)
 
 // Compaction workspace:
 // header | status[max(fused tiles, scan tiles)] | u32 counts[chunks] |
-// u64 offsets[chunks] | u32 tile counts[tiles] | u64 group sums[groups].
-// The pipelines share the header and the status words.
+// u64 offsets[chunks] | u32 tile counts[tiles] | u64 group sums[groups] |
+// u32 unit offsets[units] | u32 block totals[blocks] | u64 block prefixes[blocks].
+// The pipelines share the header and the status words.  Units (of the
+// three-phase and sparse pipelines' popcount) are min(chunk, writeout tile)
+// long; the smallest tile is 8192 / w elements, so there are at most
+// n / min(chunk, 8192 / w) of them.
 struct CompactPlan {
-    uint64_t fused_tiles, scan_tiles, chunks, groups;
+    uint64_t fused_tiles, scan_tiles, chunks, groups, units;
     uint32_t log2g;
-    size_t status_off, counts_off, offsets_off, tcounts_off, gsums_off, total;
+    size_t status_off, counts_off, offsets_off, tcounts_off, gsums_off, uoffs_off, btot_off, bpre_off, total;
 };
 
 CompactPlan plan_compact(uint64_t n, uint32_t w, uint32_t chunk_len)
@@ -147,7 +151,12 @@ CompactPlan plan_compact(uint64_t n, uint32_t w, uint32_t chunk_len)
     p.offsets_off = align_up(p.counts_off + p.chunks * 4);
     p.tcounts_off = align_up(p.offsets_off + p.chunks * 8);
     p.gsums_off = align_up(p.tcounts_off + p.fused_tiles * 4);
-    p.total = align_up(p.gsums_off + p.groups * 8);
+    const uint64_t umin = std::min<uint64_t>(chunk_len, 8192 / w);
+    p.units = (n + umin - 1) / umin;
+    p.uoffs_off = align_up(p.gsums_off + p.groups * 8);
+    p.btot_off = align_up(p.uoffs_off + p.units * 4);
+    p.bpre_off = align_up(p.btot_off + 4 * csx::popb_blocks(n));
+    p.total = align_up(p.bpre_off + 8 * csx::popb_blocks(n));
     return p;
 }
 
@@ -648,19 +657,26 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
                 "csx_compact two-pass writeout");
     }
     else {
-        auto counts = reinterpret_cast<uint32_t*>(base + p.counts_off);
-        auto offsets = reinterpret_cast<uint64_t*>(base + p.offsets_off);
-        const uint32_t log2c = log2u(chunk_len);
-        CSX_HIP(csx::launch_popc(d_mask, counts, n, log2c, bit_order, stream, hdr, status, p.scan_tiles), "csx_compact popc");
-        CSX_HIP(csx::launch_scan(counts, offsets, p.chunks, count_dst, hdr, status, stream, true), "csx_compact scan");
+        // phases 1 + 2: per block of the column, its selected count and the
+        // offsets of its units (min(chunk, writeout tile) elements) within
+        // it; then the blocks' exclusive prefixes and the total count
+        auto uoffs = reinterpret_cast<uint32_t*>(base + p.uoffs_off);
+        auto btotals = reinterpret_cast<uint32_t*>(base + p.btot_off);
+        auto bprefix = reinterpret_cast<uint64_t*>(base + p.bpre_off);
+        const uint64_t te = pipeline == CSX_PIPELINE_SPARSE ? csx::kSparseStepElems : 256ull * vpt * (16 / elem_size);
+        const uint32_t log2unit = std::min(log2u(chunk_len), log2u(te));
+        const uint32_t log2b = csx::popb_log2(n);
+        CSX_HIP(csx::launch_popc_blocks(d_mask, n, log2b, log2unit, uoffs, btotals, bit_order, stream, hdr), "csx_compact popc");
+        CSX_HIP(csx::launch_block_prefix(btotals, csx::popb_blocks(n), bprefix, count_dst, n >= (1ull << 32), stream), "csx_compact block prefixes");
         csx::TileArgs a{};
         a.in = in;
         a.mask = d_mask;
         a.out = out;
         a.n = n;
-        a.offsets = offsets;
-        a.counts = counts;
-        a.log2c = log2c;
+        a.bprefix = bprefix;
+        a.uoffs = uoffs;
+        a.log2b = log2b;
+        a.log2c = log2unit;
         if (pipeline == CSX_PIPELINE_SPARSE) {
             const bool mask_aligned = ((uintptr_t)d_mask & 15) == 0;
             CSX_HIP(timed(stream, [&] { return dispatch_sparse(elem_size, bit_order, mask_aligned, a, stream); }), "csx_compact sparse writeout");

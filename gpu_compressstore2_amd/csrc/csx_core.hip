@@ -100,6 +100,92 @@ hipError_t launch_popc(const uint8_t* mask, uint32_t* counts, uint64_t n, uint32
     return hipGetLastError();
 }
 
+hipError_t launch_popc_blocks(const uint8_t* mask, uint64_t n, uint32_t log2b, uint32_t log2u, uint32_t* uoffs, uint32_t* btotals, int order,
+                              hipStream_t s, WsHeader* hdr)
+{
+    const uint64_t blocks = (n + (1ull << log2b) - 1) >> log2b;
+    if (blocks == 0) return hipSuccess;
+    const bool aligned = ((uintptr_t)mask & 15) == 0;
+    const unsigned g = (unsigned)blocks;
+    if (order == kBitMsb) {
+        if (aligned)
+            k_popc_blocks<kBitMsb, true><<<g, kPopbThreads, 0, s>>>(mask, n, log2b, log2u, uoffs, btotals, hdr);
+        else
+            k_popc_blocks<kBitMsb, false><<<g, kPopbThreads, 0, s>>>(mask, n, log2b, log2u, uoffs, btotals, hdr);
+    }
+    else {
+        if (aligned)
+            k_popc_blocks<kBitLsb, true><<<g, kPopbThreads, 0, s>>>(mask, n, log2b, log2u, uoffs, btotals, hdr);
+        else
+            k_popc_blocks<kBitLsb, false><<<g, kPopbThreads, 0, s>>>(mask, n, log2b, log2u, uoffs, btotals, hdr);
+    }
+    return hipGetLastError();
+}
+
+// K2: bprefix[b] = sum of btotals[0..b) for b < nb, *count = the sum of all
+// (nullable).  One workgroup; thread t owns the contiguous run of `per`
+// totals from t * per, all its loads issued together.  The column's total is
+// below 2^32 unless WIDE, so the scan runs on 32-bit DPP wave scans with the
+// 16 wave sums through LDS; WIDE scans 64-bit sums.
+constexpr int kBlockPrefixThreads = 1024;
+constexpr int kBlockPrefixMaxPer = (int)(kPopbMaxBlocks / kBlockPrefixThreads);
+template <bool WIDE>
+__global__ __launch_bounds__(kBlockPrefixThreads) void k_block_prefix(const uint32_t* __restrict__ btotals, uint32_t nb,
+                                                                      uint64_t* __restrict__ bprefix, uint64_t* __restrict__ count)
+{
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nb + kBlockPrefixThreads - 1) / kBlockPrefixThreads;
+    const uint32_t b0 = t * per;
+    uint32_t v[kBlockPrefixMaxPer];
+#pragma unroll
+    for (int k = 0; k < kBlockPrefixMaxPer; k++) v[k] = 0;
+#pragma unroll
+    for (int k = 0; k < kBlockPrefixMaxPer; k++)
+        if ((uint32_t)k < per && b0 + k < nb) v[k] = btotals[b0 + k];
+    uint64_t run, total;
+    if constexpr (WIDE) {
+        __shared__ uint64_t scr[kBlockPrefixThreads / kWave];
+        uint64_t s = 0;
+#pragma unroll
+        for (int k = 0; k < kBlockPrefixMaxPer; k++) s += v[k];
+        run = block_exclusive_sum<kBlockPrefixThreads>(s, scr, &total);
+    }
+    else {
+        __shared__ uint32_t wsum[kBlockPrefixThreads / kWave];
+        uint32_t s = 0;
+#pragma unroll
+        for (int k = 0; k < kBlockPrefixMaxPer; k++) s += v[k];
+        const uint32_t inc = wave_inclusive_sum_u32(s);
+        if (lane_id() == kWave - 1) wsum[t / kWave] = inc;
+        __syncthreads();
+        uint32_t before = 0, all = 0;
+#pragma unroll
+        for (int k = 0; k < kBlockPrefixThreads / kWave; k++) {
+            const uint32_t x = wsum[k];
+            before += (uint32_t)k < t / kWave ? x : 0u;
+            all += x;
+        }
+        run = before + inc - s;
+        total = all;
+    }
+#pragma unroll
+    for (int k = 0; k < kBlockPrefixMaxPer; k++) {
+        if ((uint32_t)k < per && b0 + k < nb) bprefix[b0 + k] = run;
+        run += v[k];
+    }
+    if (t == 0 && count) *count = total;
+}
+
+hipError_t launch_block_prefix(const uint32_t* btotals, uint64_t blocks, uint64_t* bprefix, uint64_t* count, bool wide, hipStream_t s)
+{
+    if (blocks == 0 || blocks > kPopbMaxBlocks) return hipErrorInvalidValue;
+    if (wide)
+        k_block_prefix<true><<<1, kBlockPrefixThreads, 0, s>>>(btotals, (uint32_t)blocks, bprefix, count);
+    else
+        k_block_prefix<false><<<1, kBlockPrefixThreads, 0, s>>>(btotals, (uint32_t)blocks, bprefix, count);
+    return hipGetLastError();
+}
+
 hipError_t launch_lookback_init(WsHeader* hdr, uint64_t* status, uint64_t tiles, hipStream_t s)
 {
     k_lookback_init<<<grid_for(tiles, 256, 1024), 256, 0, s>>>(hdr, status, tiles);

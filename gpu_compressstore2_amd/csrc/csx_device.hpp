@@ -112,6 +112,30 @@ template <typename V> __device__ __forceinline__ V wave_inclusive_sum(V v)
     return v;
 }
 
+// 32-bit inclusive prefix sum across the 64 lanes of a wave from DPP row
+// operations (no LDS traffic): a scan within each row of 16 lanes by
+// row_shr 1, 2, 4, 8, then row_bcast:15 adds lane 15 of rows 0 and 2 to rows
+// 1 and 3, and row_bcast:31 adds lane 31 to rows 2 and 3.  A lane whose DPP
+// source is outside its row, or whose row is masked off, adds the `old`
+// operand, 0.  All 64 lanes must be active.
+__device__ __forceinline__ uint32_t wave_inclusive_sum_u32(uint32_t v)
+{
+    constexpr int kRowShr = 0x110, kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 1, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 2, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 4, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowShr + 8, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowBcast15, 0xa, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kRowBcast31, 0xc, 0xf, false);
+    return v;
+}
+
+// the wave's total, in every lane (lane 63 of the inclusive scan)
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_inclusive_sum_u32(v), kWave - 1);
+}
+
 template <typename V> __device__ __forceinline__ V wave_sum(V v)
 {
 #pragma unroll

@@ -22,6 +22,15 @@ csx_status compact_in_workspace(const void* d_in, const uint8_t* d_mask, void* d
 hipError_t launch_popc(const uint8_t* mask, uint32_t* counts, uint64_t n, uint32_t log2c, int order, hipStream_t s, WsHeader* hdr = nullptr,
                        uint64_t* status = nullptr, uint64_t init_tiles = 0);
 inline uint64_t scan_tiles(uint64_t m) { return (m + kScanTileElems - 1) / kScanTileElems; }
+// The pipelines' popcount: one workgroup per block of 2^log2b elements writes
+// the block's selected count to btotals and every 2^log2u-element unit's
+// exclusive offset within its block to uoffs (csx_kernels.hpp k_popc_blocks);
+// clears hdr->error when hdr is set.  Then one workgroup turns the block
+// counts into exclusive block prefixes and writes the total to *count
+// (nullable); blocks <= kPopbMaxBlocks; `wide`: the total may reach 2^32.
+hipError_t launch_popc_blocks(const uint8_t* mask, uint64_t n, uint32_t log2b, uint32_t log2u, uint32_t* uoffs, uint32_t* btotals, int order,
+                              hipStream_t s, WsHeader* hdr = nullptr);
+hipError_t launch_block_prefix(const uint32_t* btotals, uint64_t blocks, uint64_t* bprefix, uint64_t* count, bool wide, hipStream_t s);
 hipError_t launch_lookback_init(WsHeader* hdr, uint64_t* status, uint64_t tiles, hipStream_t s);
 // initialized: the look-back state was already zeroed (by launch_popc)
 hipError_t launch_scan(const uint32_t* counts, uint64_t* offsets, uint64_t m, uint64_t* total, WsHeader* hdr, uint64_t* status, hipStream_t s,

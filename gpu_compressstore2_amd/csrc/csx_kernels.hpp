@@ -195,6 +195,134 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const uint32_t* counts, Out
 }
 
 // ======================================================================
+// phases 1 + 2 of the compaction pipelines: block totals, unit offsets,
+// block prefixes
+// ======================================================================
+//
+// K1 (k_popc_blocks).  The column is cut into blocks of BE = 2^log2b
+// elements (2^17 up to 2^32 elements, more beyond so that there are at most
+// kPopbMaxBlocks), one 256-thread workgroup each, taken in rounds of 2^17
+// elements = 1024 mask vectors (16 bytes, 128 elements each).  In a round,
+// wave w owns vectors [256 w, 256 w + 256): its load j reads vector
+// 256 w + 64 j + lane, so every wave load reads 1 KiB contiguous, and the
+// four loads of the next round are issued before the current round is
+// processed.  A DPP scan of each load's popcounts, the four loads' totals
+// added in order, and one exchange of the four wave totals through LDS give
+// every unit of U = 2^log2u elements (32 <= U) its exclusive offset within
+// the block: uoffs[unit] is written by the lane that holds the unit's first
+// element.  The block's selected count goes to btotals[b].
+//
+// K2 (k_block_prefix).  One workgroup turns the block counts into exclusive
+// block prefixes (u64) and writes the total selected count.
+//
+// No workgroup waits for another, so there is no look-back state to
+// initialise; a writeout tile's output offset is bprefix[its block] +
+// uoffs[its first unit], two words it requests before its mask.  Bits at
+// element index >= n do not count, whatever the buffer holds.
+constexpr int kPopbThreads = 256;
+constexpr int kPopbLoads = 4;  // vectors per thread and round
+constexpr int kPopbWaves = kPopbThreads / kWave;
+static_assert((1u << kPopbMinLog2) == kPopbThreads * kPopbLoads * 128, "a round is one 2^17-element block");
+
+// `hdr` (nullable): its error word is cleared, so that the workspace error
+// query answers for this call too (no kernel of these pipelines sets it).
+template <int ORDER, bool ALIGNED>
+__global__ __launch_bounds__(kPopbThreads) void k_popc_blocks(const uint8_t* __restrict__ mask, uint64_t n, uint32_t log2b, uint32_t log2u,
+                                                              uint32_t* __restrict__ uoffs, uint32_t* __restrict__ btotals, WsHeader* hdr)
+{
+    __shared__ uint32_t s_wsum[2][kPopbWaves];
+    if (hdr && blockIdx.x == 0 && threadIdx.x == 0) hdr->error = 0;
+    const uint32_t t = threadIdx.x, lane = lane_id(), wid = t / kWave;
+    const uint64_t nbytes = (n + 7) / 8;
+    const uint64_t e_block = (uint64_t)blockIdx.x << log2b;
+    const uint64_t e_end = min(n, e_block + (1ull << log2b));
+    const uint64_t nv = (e_end - e_block + 127) / 128;  // vectors holding elements of this block
+    const uint32_t rounds = (uint32_t)((nv + 1023) / 1024);
+    const uint8_t* bmask = mask + e_block / 8;
+    const uint64_t bbytes = nbytes - e_block / 8;
+    // vector of load j in round r (block-relative)
+    auto vec_of = [&](uint32_t r, int j) { return (uint64_t)r * 1024 + wid * 256 + j * kWave + lane; };
+    u32x4 x[kPopbLoads];
+#pragma unroll
+    for (int j = 0; j < kPopbLoads; j++) x[j] = vec_of(0, j) < nv ? load_mask16<ALIGNED>(bmask, vec_of(0, j) * 16, bbytes) : u32x4{0, 0, 0, 0};
+    uint32_t carry = 0;  // selected elements of the block in earlier rounds
+    for (uint32_t r = 0; r < rounds; r++) {
+        uint32_t w[kPopbLoads][4];
+#pragma unroll
+        for (int j = 0; j < kPopbLoads; j++) {
+            w[j][0] = to_lsb_word<ORDER>(x[j].x);
+            w[j][1] = to_lsb_word<ORDER>(x[j].y);
+            w[j][2] = to_lsb_word<ORDER>(x[j].z);
+            w[j][3] = to_lsb_word<ORDER>(x[j].w);
+        }
+        // the next round's loads, in flight while this round is scanned
+        if (r + 1 < rounds) {
+#pragma unroll
+            for (int j = 0; j < kPopbLoads; j++)
+                x[j] = vec_of(r + 1, j) < nv ? load_mask16<ALIGNED>(bmask, vec_of(r + 1, j) * 16, bbytes) : u32x4{0, 0, 0, 0};
+        }
+        uint32_t ex[kPopbLoads];
+        uint32_t wave_total = 0;  // uniform
+#pragma unroll
+        for (int j = 0; j < kPopbLoads; j++) {
+            const uint64_t e0 = e_block + vec_of(r, j) * 128;
+            if (e0 + 128 > n) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint64_t b = e0 + 32 * k;
+                    w[j][k] = b >= n ? 0u : low_bits(w[j][k], (uint32_t)(n - b));
+                }
+            }
+            const uint32_t c = popc32(w[j][0]) + popc32(w[j][1]) + popc32(w[j][2]) + popc32(w[j][3]);
+            const uint32_t inc = wave_inclusive_sum_u32(c);
+            ex[j] = wave_total + inc - c;
+            wave_total += (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
+        }
+        // two LDS buffers: a round's wave totals are overwritten two rounds
+        // later, after every thread has passed the barrier between
+        if (lane == 0) s_wsum[r & 1][wid] = wave_total;
+        __syncthreads();
+        uint32_t before = 0, all = 0;
+#pragma unroll
+        for (int k = 0; k < kPopbWaves; k++) {
+            const uint32_t s = s_wsum[r & 1][k];
+            before += (uint32_t)k < wid ? s : 0u;
+            all += s;
+        }
+#pragma unroll
+        for (int j = 0; j < kPopbLoads; j++) {
+            const uint64_t e0 = e_block + vec_of(r, j) * 128;
+            if (e0 >= n) continue;
+            const uint32_t o = carry + before + ex[j];
+            if (log2u >= 7) {
+                if ((e0 & ((1ull << log2u) - 1)) == 0) uoffs[e0 >> log2u] = o;
+            }
+            else {
+                const uint32_t p0 = popc32(w[j][0]), p1 = popc32(w[j][1]), p2 = popc32(w[j][2]);
+                if (log2u == 6) {
+                    const uint64_t u = e0 >> 6;
+                    uoffs[u] = o;
+                    if (e0 + 64 < n) uoffs[u + 1] = o + p0 + p1;
+                }
+                else {
+                    const uint64_t u = e0 >> 5;
+                    if (e0 + 128 <= n) {
+                        *reinterpret_cast<u32x4*>(uoffs + u) = u32x4{o, o + p0, o + p0 + p1, o + p0 + p1 + p2};
+                    }
+                    else {
+                        const uint32_t q[4] = {o, o + p0, o + p0 + p1, o + p0 + p1 + p2};
+                        for (int k = 0; k < 4; k++)
+                            if (e0 + 32 * k < n) uoffs[u + k] = q[k];
+                    }
+                }
+            }
+        }
+        carry += all;
+    }
+    if (t == 0) btotals[blockIdx.x] = carry;
+}
+
+// ======================================================================
 // phase 3 / fused: the compaction tile engine
 // ======================================================================
 //
@@ -229,6 +357,8 @@ template <typename C> struct TileShared {
     uint32_t words[C::NWORDS];
     uint32_t wpre[C::NWORDS];
     uint64_t scr[C::BLOCK / kWave];
+    uint32_t wtot[C::BLOCK / kWave];
+    uint32_t count;
     uint64_t tile;
     uint64_t off;
 };
@@ -317,6 +447,17 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         for (int k = 0; k < 8; k++) psum += acc[k];
     }
 
+    // three-phase writeout with two-level offsets: the tile's block prefix
+    // and its unit's offset, requested ahead of the mask
+    uint64_t bpre = 0;
+    uint32_t uoff = 0;
+    if constexpr (MODE == kModeWriteout) {
+        if (a.bprefix && t == 0) {
+            bpre = a.bprefix[tbase >> a.log2b];
+            uoff = a.uoffs[tbase >> a.log2c];
+        }
+    }
+
     // ---- mask words, word prefixes, tile count ----
     uint32_t w[4] = {0, 0, 0, 0};
     uint32_t wsum = 0;
@@ -352,22 +493,57 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         }
         wsum = popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
     }
-    // one block scan of (psum << 16 | wsum): the low field (a tile holds at
-    // most 32768 elements) never carries into the high one
-    uint64_t packed_total;
-    const uint64_t pex = block_exclusive_sum<C::BLOCK>((psum << 16) | wsum, sh.scr, &packed_total);
-    const uint64_t wex = pex & 0xFFFF;
-    const uint64_t tile_count = packed_total & 0xFFFF;
-    const uint64_t prefix_total = packed_total >> 16;
-    (void)prefix_total;
+    // Scan of the mask threads' popcounts: each mask word's prefix in the
+    // tile, and the tile count.  Two-pass folds its offset partial sums into
+    // a 64-bit block scan of (psum << 16 | wsum) (the low field, at most 32768,
+    // never carries into the high one).  The other modes scan 32 bits with DPP
+    // row operations: in one wave when the tile's mask fits in 64 threads
+    // (then one barrier publishes words, prefixes and count together), else in
+    // every wave with the wave totals through LDS.
+    uint32_t wex = 0, tile_count = 0;
+    uint64_t prefix_total = 0;
+    bool published = false;  // words, prefixes and count already visible to the block
+    if constexpr (MODE == kModeTwoPass) {
+        uint64_t packed_total;
+        const uint64_t pex = block_exclusive_sum<C::BLOCK>((psum << 16) | wsum, sh.scr, &packed_total);
+        wex = (uint32_t)(pex & 0xFFFF);
+        tile_count = (uint32_t)(packed_total & 0xFFFF);
+        prefix_total = packed_total >> 16;
+    }
+    else if constexpr (C::MASK_THREADS <= kWave) {
+        if (t < (uint32_t)kWave) {
+            const uint32_t inc = wave_inclusive_sum_u32(wsum);
+            wex = inc - wsum;
+            if (t == kWave - 1) sh.count = inc;
+        }
+    }
+    else {
+        const uint32_t inc = wave_inclusive_sum_u32(wsum);
+        if (lane_id() == kWave - 1) sh.wtot[t / kWave] = inc;
+        __syncthreads();
+        uint32_t before = 0, all = 0;
+#pragma unroll
+        for (int k = 0; k < C::MASK_THREADS / kWave; k++) {
+            const uint32_t s = sh.wtot[k];
+            before += (uint32_t)k < t / kWave ? s : 0u;
+            all += s;
+        }
+        wex = before + inc - wsum;
+        tile_count = all;
+    }
     if (t < (uint32_t)C::MASK_THREADS) {
-        uint32_t run = (uint32_t)wex;
+        uint32_t run = wex;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             sh.words[4 * t + k] = w[k];
             sh.wpre[4 * t + k] = run;
             run += popc32(w[k]);
         }
+    }
+    if constexpr (MODE != kModeTwoPass && C::MASK_THREADS <= kWave) {
+        __syncthreads();
+        tile_count = sh.count;
+        published = true;
     }
     if constexpr (MODE != kModeFused) {
         // an empty tile has nothing to read or write (the fused form must
@@ -379,7 +555,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
             return;
         }
     }
-    __syncthreads();
+    if (!published) __syncthreads();
 
     // ---- data loads (predicated on the vector's mask bits) ----
     u32x4 vec[C::VPT];
@@ -456,6 +632,10 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
                 if (tile == ntiles - 1 && a.count_out) *a.count_out = pre + tile_count;
             }
         }
+        __syncthreads();
+    }
+    else if (a.bprefix) {
+        if (t == 0) sh.off = bpre + uoff;
         __syncthreads();
     }
     else {
@@ -669,7 +849,7 @@ __global__ __launch_bounds__(256) void k_thread_per_chunk(TileArgs a)
 // sized for a fully selected tile: below a few percent selectivity, where the
 // staged tile waits on load latency, this form is faster; above, the
 // per-element gathers lose to whole-vector loads.
-constexpr uint64_t kSparseStep = 64 * 128;
+constexpr uint64_t kSparseStep = kSparseStepElems;
 
 template <int W, int ORDER, bool ALIGNED>
 __global__ __launch_bounds__(256) void k_sparse(TileArgs a)
@@ -685,6 +865,14 @@ __global__ __launch_bounds__(256) void k_sparse(TileArgs a)
     for (uint64_t step = (blockIdx.x * 256ull + threadIdx.x) / kWave; step < steps; step += waves) {
         const uint64_t e0 = step * kSparseStep;
         const uint64_t el = e0 + 128ull * lane;
+        // two-level offsets: the step's block prefix and unit offset,
+        // requested ahead of the mask
+        uint64_t bpre = 0;
+        uint32_t uoff = 0;
+        if (a.bprefix) {
+            bpre = a.bprefix[e0 >> a.log2b];
+            uoff = a.uoffs[e0 >> a.log2c];
+        }
         uint32_t w[4] = {0, 0, 0, 0};
         if (el < n) {
             const u32x4 x = load_mask16<ALIGNED>(a.mask, el / 8, nbytes);
@@ -700,12 +888,17 @@ __global__ __launch_bounds__(256) void k_sparse(TileArgs a)
                 }
             }
         }
-        // output offset of the step: its chunk's offset, plus the selected
-        // elements of the chunk before the step when the chunk is longer
+        // output offset of the step: with two-level offsets the totals of
+        // the blocks before it plus its unit's offset (the unit divides the
+        // step); else its chunk's offset, plus the selected elements of the
+        // chunk before the step when the chunk is longer
         const uint64_t chunk = e0 >> a.log2c;
         const uint64_t cstart = chunk << a.log2c;
-        uint64_t before = 0;
-        if (cstart < e0) {
+        uint64_t before = 0, step_off = 0;
+        if (a.bprefix) {
+            step_off = bpre + uoff;
+        }
+        else if (cstart < e0) {
             // whole 16-byte vectors below e0 < n: no tail bits among them
             uint32_t c = 0;
             for (uint64_t v = cstart / 128 + lane; v < e0 / 128; v += kWave) {
@@ -714,8 +907,9 @@ __global__ __launch_bounds__(256) void k_sparse(TileArgs a)
             }
             before = wave_sum((uint64_t)c);
         }
+        if (!a.bprefix) step_off = (a.offsets ? a.offsets[chunk] : (uint64_t)a.offsets32[chunk]) + before;
         const uint32_t cnt = popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
-        uint64_t r = base_out + (a.offsets ? a.offsets[chunk] : (uint64_t)a.offsets32[chunk]) + before + wave_inclusive_sum(cnt) - cnt;
+        uint64_t r = base_out + step_off + wave_inclusive_sum_u32(cnt) - cnt;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             uint32_t bits = w[k];

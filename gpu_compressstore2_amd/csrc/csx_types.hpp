@@ -22,6 +22,22 @@ template <int W> constexpr uint64_t kTileElems = 256ull * kTileVpt<W> * (16ull /
 // Elements per tile of the chunk-count scan.
 constexpr uint64_t kScanTileElems = 256ull * 16ull;
 
+// Elements per wave step of the sparse writeout (64 lanes x 128 elements).
+constexpr uint64_t kSparseStepElems = 64ull * 128ull;
+
+// Blocks of the pipelines' popcount (k_popc_blocks) for n elements: log2 of
+// the block length, 2^17 elements while that makes at most kPopbMaxBlocks
+// blocks (n <= 2^32), longer beyond (at most 2^25 elements for n <= 2^40).
+constexpr uint32_t kPopbMinLog2 = 17;
+constexpr uint32_t kPopbMaxBlocks = 32768;
+inline uint32_t popb_log2(uint64_t n)
+{
+    uint32_t l = kPopbMinLog2;
+    while (((n + (1ull << l) - 1) >> l) > kPopbMaxBlocks) l++;
+    return l;
+}
+inline uint64_t popb_blocks(uint64_t n) { return (n + (1ull << popb_log2(n)) - 1) >> popb_log2(n); }
+
 // kModeGiven: the tile's output offset is supplied in TileArgs::given_off
 // kModePattern: no mask in memory; mask words and the tile's offset follow
 // from a periodic pattern (TileArgs::pat_*)
@@ -39,6 +55,15 @@ struct TileArgs {
     const uint32_t* counts;     // nullable: skip tiles whose chunks are empty
     uint32_t log2c;
     const uint64_t* out_base;   // nullable: device value added to every output offset
+    // two-level offsets of the pipelines (k_popc_blocks + k_block_prefix),
+    // used instead of offsets/offsets32 when bprefix is set: a tile's (or
+    // sparse step's) output offset is bprefix[first element >> log2b] (the
+    // selected count of the blocks before its own) plus
+    // uoffs[first element >> log2c] (its unit's offset within the block;
+    // 2^log2c divides the tile)
+    const uint64_t* bprefix;
+    const uint32_t* uoffs;
+    uint32_t log2b;
     // fused
     WsHeader* hdr;
     uint64_t* status;
