@@ -103,6 +103,7 @@ SIGNATURES = {
     "csx_gen_values": (_int, [_vp, _u64, _u32, _int, _u64, _vp]),
     "csx_pattern_count": (_int, [_u64, _u32, _u32, ctypes.POINTER(_u64)]),
     "csx_pattern_compact": (_int, [_vp, _vp, _u64, _u32, _u32, _u32, _vp, ctypes.POINTER(_u64), _vp]),
+    "csx_stream_probe": (_int, [_int, _vp, _vp, _u64, _u32, _u32, _vp]),
     "csx_profile_begin": (_int, []),
     "csx_profile_end": (_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
     "csx_compact_config": (_int, [_vp, _vp, _vp, _u64, _u32, _int, _vp, _vp, _vp, _vp, _sz, _vp]),
@@ -655,6 +656,16 @@ def copy(dst, src, nbytes=None, stream=None):
     nbytes = src.numel() * src.element_size() if nbytes is None else nbytes
     _check(lib().csx_copy(_ptr(dst), _ptr(src), nbytes, _stream(stream)))
     return dst
+
+
+PROBE_READ, PROBE_WRITE, PROBE_COPY, PROBE_MIX = 0, 1, 2, 3
+
+
+def stream_probe(kind, dst, src, nbytes, vectors_per_thread=1, xcd_run=0, stream=None):
+    """csx_stream_probe: a plain read / write / copy / 2:1 mix kernel over
+    nbytes in the writeout's access shape (the benchmark's HBM reference)."""
+    _require_device(dst, src)
+    _check(lib().csx_stream_probe(kind, _ptr(dst), _ptr(src), nbytes, vectors_per_thread, xcd_run, _stream(stream)))
 
 
 def profile_begin():

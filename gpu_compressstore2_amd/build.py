@@ -85,13 +85,11 @@ DRIVER = os.path.join(PKG, "csx_bench")
 
 
 SERVE = os.path.join(PKG, "csx_serve")
-HBM_PROBE = os.path.join(PKG, "csx_hbm_probe")
 
 
 def _build_driver():
-    """The main.cu-compatible benchmark driver, the serving-loop benchmark and the HBM probe."""
-    for src, target, libs in (("csx_bench.cpp", DRIVER, ["-lcsx", "-lcsx_baseline"]), ("csx_serve.cpp", SERVE, ["-lcsx"]),
-                              ("csx_hbm_probe.hip", HBM_PROBE, [])):
+    """The main.cu-compatible benchmark driver and the serving-loop benchmark."""
+    for src, target, libs in (("csx_bench.cpp", DRIVER, ["-lcsx", "-lcsx_baseline"]), ("csx_serve.cpp", SERVE, ["-lcsx"])):
         tmp = target + ".tmp"
         lang = ["-x", "hip", *DEVICE] if src.endswith(".hip") else [f"--offload-arch={ARCH}"]
         cmd = [HIPCC, *lang, *COMMON, "-o", tmp, os.path.join(PKG, "driver", src), f"-L{PKG}", *libs, "-Wl,-rpath,$ORIGIN"]

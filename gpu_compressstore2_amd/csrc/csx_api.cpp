@@ -229,8 +229,44 @@ struct Profiler {
 thread_local Profiler g_prof;
 constexpr size_t kMaxProfiled = 1u << 16;
 
-// Launch `fn` (returning hipError_t) between a recorded event pair when profiling.
+} // namespace
+namespace csx {
+thread_local LaunchEvents g_launch_events;
+}
+namespace {
+
+// When profiling: `fn` (returning hipError_t) launches exactly one kernel
+// through launch_k, whose start and end the armed event pair records.  A call
+// that launches nothing (an empty input) leaves no pair.
 template <typename F> hipError_t timed(hipStream_t s, F&& fn)
+{
+    (void)s;
+    if (!g_prof.on || g_prof.events.size() >= kMaxProfiled) return fn();
+    hipEvent_t a, b;
+    hipError_t e = hipEventCreate(&a);
+    if (e != hipSuccess) return e;
+    e = hipEventCreate(&b);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return e;
+    }
+    csx::g_launch_events = csx::LaunchEvents{a, b, false};
+    e = fn();
+    const bool used = csx::g_launch_events.used;
+    csx::g_launch_events = csx::LaunchEvents{};
+    if (used) {
+        g_prof.events.emplace_back(a, b);
+    }
+    else {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    return e;
+}
+
+// When profiling: `fn` (several kernels) between an event pair recorded on
+// the stream before and after it.
+template <typename F> hipError_t timed_span(hipStream_t s, F&& fn)
 {
     if (!g_prof.on || g_prof.events.size() >= kMaxProfiled) return fn();
     hipEvent_t a, b;
@@ -985,7 +1021,7 @@ static csx_status batch_impl(const csx_segment* d_segments, uint32_t nseg, uint6
     if (!launch_fits(p.max_tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)p.max_tiles);
     auto base = static_cast<uint8_t*>(d_ws);
     const uint32_t count_blocks = cfg.count_blocks ? cfg.count_blocks : 4096;
-    CSX_HIP(timed(stream, [&] {
+    CSX_HIP(timed_span(stream, [&] {
                 return csx::launch_batch(elem_size, bit_order, reinterpret_cast<const csx::Segment*>(d_segments), nseg, p.max_tiles, d_counts,
                                          reinterpret_cast<uint64_t*>(base + p.prefix_off), reinterpret_cast<uint32_t*>(base + p.tcounts_off),
                                          reinterpret_cast<uint64_t*>(base + p.goff_off), reinterpret_cast<csx::WsHeader*>(base),
@@ -1125,6 +1161,22 @@ csx_status csx_copy(void* d_dst, const void* d_src, uint64_t bytes, hipStream_t 
     if (!d_dst || !d_src) return fail(CSX_ERR_INVALID, "null pointer");
     if ((((uintptr_t)d_dst) | ((uintptr_t)d_src)) & 15) return fail(CSX_ERR_INVALID, "csx_copy needs 16-byte aligned pointers");
     CSX_HIP(timed(stream, [&] { return csx::launch_copy(d_dst, d_src, bytes, stream); }), "csx_copy launch");
+    return ok();
+}
+
+csx_status csx_stream_probe(int kind, void* d_dst, const void* d_src, uint64_t bytes, uint32_t vectors_per_thread, uint32_t xcd_run,
+                            hipStream_t stream)
+{
+    if (kind < CSX_PROBE_READ || kind > CSX_PROBE_MIX) return fail(CSX_ERR_INVALID, "unknown probe kind %d", kind);
+    if (vectors_per_thread != 1 && vectors_per_thread != 4) return fail(CSX_ERR_INVALID, "vectors per thread must be 1 or 4");
+    if (xcd_run > 64) return fail(CSX_ERR_INVALID, "XCD run of %u blocks (at most 64)", xcd_run);
+    if (bytes % 32) return fail(CSX_ERR_INVALID, "probe size must be a multiple of 32 bytes");
+    if (bytes / 16 / (256ull * vectors_per_thread) >= (1ull << 31)) return fail(CSX_ERR_UNSUPPORTED, "probe too large for one launch");
+    if (bytes == 0) return ok();
+    if (!d_dst || (kind != CSX_PROBE_WRITE && !d_src)) return fail(CSX_ERR_INVALID, "null pointer");
+    if (((uintptr_t)d_dst | (uintptr_t)d_src) & 15) return fail(CSX_ERR_INVALID, "probe pointers must be 16-byte aligned");
+    CSX_HIP(timed(stream, [&] { return csx::launch_stream_probe(kind, d_src, d_dst, bytes, vectors_per_thread, xcd_run, stream); }),
+            "csx_stream_probe launch");
     return ok();
 }
 

@@ -265,8 +265,97 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
 {
     const uint64_t nv = bytes / 16;
     const uint32_t tail = (uint32_t)(bytes % 16);
-    k_copy<<<grid_for(nv, 256, 1024), 256, 0, s>>>(static_cast<const u32x4*>(src), static_cast<u32x4*>(dst), nv,
-                                                  static_cast<const uint8_t*>(src) + nv * 16, static_cast<uint8_t*>(dst) + nv * 16, tail);
+    launch_k(k_copy, dim3(grid_for(nv, 256, 1024)), dim3(256), s, static_cast<const u32x4*>(src), static_cast<u32x4*>(dst), nv,
+             static_cast<const uint8_t*>(src) + nv * 16, static_cast<uint8_t*>(dst) + nv * 16, tail);
+    return hipGetLastError();
+}
+
+// Streaming probes: what HBM sustains for plain access mixes with the
+// writeout's own shape -- a grid of one block per 256 * U 16-byte vectors
+// (no loop), non-temporal hints, blocks optionally in XCD runs (xcd_run > 0:
+// the order of the writeout tiles).  Kind 0 reads (one word of the reads'
+// xor is stored only if it matches a constant), 1 writes, 2 copies, 3 reads
+// two vectors per vector written (the 2:1 byte mix of a 50 % compaction).
+static __device__ __forceinline__ uint64_t probe_block(uint32_t xcd_run)
+{
+    const uint64_t b = blockIdx.x, n = gridDim.x;
+    if (!xcd_run) return b;
+    const uint64_t grp = kXcds * xcd_run, g = b / grp;
+    if ((g + 1) * grp > n) return b;
+    return g * grp + (b % kXcds) * xcd_run + (b / kXcds) % xcd_run;
+}
+
+template <int KIND, int U>
+static __global__ __launch_bounds__(256) void k_stream_probe(const u32x4* __restrict__ src, u32x4* __restrict__ dst, uint64_t nv, uint32_t xcd_run)
+{
+    const uint64_t base = probe_block(xcd_run) * 256 * U + threadIdx.x;
+    if constexpr (KIND == 0) {
+        u32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const uint64_t i = base + (uint64_t)j * 256;
+            if (i < nv) acc ^= __builtin_nontemporal_load(src + i);
+        }
+        if ((acc.x ^ acc.y ^ acc.z ^ acc.w) == 0x9E3779B9u) dst[0] = acc;
+    }
+    else if constexpr (KIND == 1) {
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const uint64_t i = base + (uint64_t)j * 256;
+            if (i < nv) __builtin_nontemporal_store(u32x4{(uint32_t)i, 1u, 2u, 3u}, dst + i);
+        }
+    }
+    else if constexpr (KIND == 2) {
+        u32x4 v[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const uint64_t i = base + (uint64_t)j * 256;
+            v[j] = i < nv ? __builtin_nontemporal_load(src + i) : u32x4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const uint64_t i = base + (uint64_t)j * 256;
+            if (i < nv) __builtin_nontemporal_store(v[j], dst + i);
+        }
+    }
+    else {
+        // output vector o <- input vectors 2 o and 2 o + 1 (nv counts outputs)
+        u32x4 a[U], c[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const uint64_t o = base + (uint64_t)j * 256;
+            a[j] = o < nv ? __builtin_nontemporal_load(src + 2 * o) : u32x4{0, 0, 0, 0};
+            c[j] = o < nv ? __builtin_nontemporal_load(src + 2 * o + 1) : u32x4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const uint64_t o = base + (uint64_t)j * 256;
+            if (o < nv) __builtin_nontemporal_store(a[j] ^ c[j], dst + o);
+        }
+    }
+}
+
+template <int KIND> static void launch_probe_kind(const u32x4* src, u32x4* dst, uint64_t nv, uint32_t vpt, uint32_t xcd_run, hipStream_t s)
+{
+    const uint64_t per = 256ull * vpt;
+    const dim3 grid((unsigned)((nv + per - 1) / per));
+    if (vpt == 4)
+        launch_k(k_stream_probe<KIND, 4>, grid, dim3(256), s, src, dst, nv, xcd_run);
+    else
+        launch_k(k_stream_probe<KIND, 1>, grid, dim3(256), s, src, dst, nv, xcd_run);
+}
+
+hipError_t launch_stream_probe(int kind, const void* src, void* dst, uint64_t bytes, uint32_t vpt, uint32_t xcd_run, hipStream_t s)
+{
+    auto in = static_cast<const u32x4*>(src);
+    auto out = static_cast<u32x4*>(dst);
+    const uint64_t nv = bytes / 16;
+    switch (kind) {
+        case 0: launch_probe_kind<0>(in, out, nv, vpt, xcd_run, s); break;
+        case 1: launch_probe_kind<1>(in, out, nv, vpt, xcd_run, s); break;
+        case 2: launch_probe_kind<2>(in, out, nv, vpt, xcd_run, s); break;
+        default: launch_probe_kind<3>(in, out, nv / 2, vpt, xcd_run, s); break;
+    }
     return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
 // csx_internal.hpp -- host-side launchers behind the C ABI (not exported).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "csx.h"
@@ -8,6 +9,27 @@
 #include "csx_types.hpp"
 
 namespace csx {
+
+// Kernel timing for csx_profile_begin: an event pair armed by the caller
+// (csx_api.cpp) is consumed by the next launch_k on this host thread, which
+// launches through hipExtLaunchKernelGGL so that the events carry the start
+// and end of that kernel itself, not of the stream around it.
+struct LaunchEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool used = false;
+};
+extern thread_local LaunchEvents g_launch_events;
+template <typename K, typename... A> inline void launch_k(K kernel, dim3 grid, dim3 block, hipStream_t s, A... args)
+{
+    LaunchEvents& ev = g_launch_events;
+    if (ev.start && !ev.used) {
+        ev.used = true;
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev.start, ev.stop, 0, args...);
+    }
+    else {
+        kernel<<<grid, block, 0, s>>>(args...);
+    }
+}
 
 // the calling thread's csx_last_error() message (csx_api.cpp)
 csx_status set_error(csx_status s, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -42,6 +64,8 @@ hipError_t launch_tree_offsets(const uint32_t* tree, uint32_t* out, uint64_t m, 
 hipError_t launch_store_u64(uint64_t* dst, uint64_t v, hipStream_t s);
 // both pointers 16-byte aligned
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);
+// csx_stream_probe's kernels (kind 0 read, 1 write, 2 copy, 3 2:1 mix)
+hipError_t launch_stream_probe(int kind, const void* src, void* dst, uint64_t bytes, uint32_t vpt, uint32_t xcd_run, hipStream_t s);
 // *dst = base[i-1] + total[i-1] (0 for i = 0): the multi-stream carry
 hipError_t launch_carry(uint64_t* dst, const uint64_t* base, const uint64_t* total, uint32_t i, hipStream_t s);
 // batched compaction (csx_batch.hip)

@@ -12,27 +12,27 @@ namespace csx {
 template <int W, int ORDER, bool ALIGNED, int VPT> static void launch_writeout_vpt(const TileArgs& a, hipStream_t s)
 {
     const uint64_t tiles = (a.n + TileCfg<W, VPT>::TE - 1) / TileCfg<W, VPT>::TE;
-    k_tile<W, ORDER, ALIGNED, kModeWriteout, true, VPT><<<(unsigned)tiles, 256, 0, s>>>(a);
+    launch_k(k_tile<W, ORDER, ALIGNED, kModeWriteout, true, VPT>, dim3((unsigned)tiles), dim3(256), s, a);
 }
 
 template <int W, bool ALIGNED, int VPT> static void launch_pattern_vpt(const TileArgs& a, hipStream_t s)
 {
     const uint64_t tiles = (a.n + TileCfg<W, VPT>::TE - 1) / TileCfg<W, VPT>::TE;
-    k_tile<W, kBitMsb, ALIGNED, kModePattern, true, VPT><<<(unsigned)tiles, 256, 0, s>>>(a);
+    launch_k(k_tile<W, kBitMsb, ALIGNED, kModePattern, true, VPT>, dim3((unsigned)tiles), dim3(256), s, a);
 }
 
 template <int W, int ORDER, bool ALIGNED>
 static void launch_tile_impl(int mode, bool staged, int vpt, const TileArgs& a, unsigned grid, hipStream_t s)
 {
     if (mode == kModeTwoPass) {
-        k_tile<W, ORDER, ALIGNED, kModeTwoPass, true><<<grid, 256, 0, s>>>(a);
+        launch_k(k_tile<W, ORDER, ALIGNED, kModeTwoPass, true>, dim3(grid), dim3(256), s, a);
     }
     else if (mode == kModeFused) {
-        k_tile<W, ORDER, ALIGNED, kModeFused, true><<<grid, 256, 0, s>>>(a);
+        launch_k(k_tile<W, ORDER, ALIGNED, kModeFused, true>, dim3(grid), dim3(256), s, a);
     }
     else if (mode == kModeGiven) {
         // one tile at a.given_off (inputs of at most one tile: the whole call)
-        k_tile<W, ORDER, ALIGNED, kModeGiven, true><<<grid, 256, 0, s>>>(a);
+        launch_k(k_tile<W, ORDER, ALIGNED, kModeGiven, true>, dim3(grid), dim3(256), s, a);
     }
     else if (mode == kModePattern) {
         // no mask to order: one instantiation per tile size
@@ -56,11 +56,11 @@ static void launch_tile_impl(int mode, bool staged, int vpt, const TileArgs& a, 
             case 16:
                 if constexpr (W >= 2) launch_writeout_vpt<W, ORDER, ALIGNED, 16>(a, s);
                 break;
-            default: k_tile<W, ORDER, ALIGNED, kModeWriteout, true><<<grid, 256, 0, s>>>(a); break;
+            default: launch_k(k_tile<W, ORDER, ALIGNED, kModeWriteout, true>, dim3(grid), dim3(256), s, a); break;
         }
     }
     else {
-        k_tile<W, ORDER, ALIGNED, kModeWriteout, false><<<grid, 256, 0, s>>>(a);
+        launch_k(k_tile<W, ORDER, ALIGNED, kModeWriteout, false>, dim3(grid), dim3(256), s, a);
     }
 }
 
@@ -114,9 +114,9 @@ template <int W> hipError_t launch_thread_per_chunk(int order, const TileArgs& a
     if (g < 1) g = 1;
     if (g > 65536) g = 65536;
     if (order == kBitMsb)
-        k_thread_per_chunk<W, kBitMsb><<<(unsigned)g, 256, 0, s>>>(a);
+        launch_k(k_thread_per_chunk<W, kBitMsb>, dim3((unsigned)g), dim3(256), s, a);
     else
-        k_thread_per_chunk<W, kBitLsb><<<(unsigned)g, 256, 0, s>>>(a);
+        launch_k(k_thread_per_chunk<W, kBitLsb>, dim3((unsigned)g), dim3(256), s, a);
     return hipGetLastError();
 }
 
@@ -128,15 +128,15 @@ template <int W> hipError_t launch_sparse(int order, bool mask_aligned, const Ti
     if (g > (1u << 22)) g = 1u << 22;  // beyond 2^37 elements the waves loop
     if (order == kBitMsb) {
         if (mask_aligned)
-            k_sparse<W, kBitMsb, true><<<(unsigned)g, 256, 0, s>>>(a);
+            launch_k(k_sparse<W, kBitMsb, true>, dim3((unsigned)g), dim3(256), s, a);
         else
-            k_sparse<W, kBitMsb, false><<<(unsigned)g, 256, 0, s>>>(a);
+            launch_k(k_sparse<W, kBitMsb, false>, dim3((unsigned)g), dim3(256), s, a);
     }
     else {
         if (mask_aligned)
-            k_sparse<W, kBitLsb, true><<<(unsigned)g, 256, 0, s>>>(a);
+            launch_k(k_sparse<W, kBitLsb, true>, dim3((unsigned)g), dim3(256), s, a);
         else
-            k_sparse<W, kBitLsb, false><<<(unsigned)g, 256, 0, s>>>(a);
+            launch_k(k_sparse<W, kBitLsb, false>, dim3((unsigned)g), dim3(256), s, a);
     }
     return hipGetLastError();
 }

@@ -472,6 +472,24 @@ csx_status csx_gen_values(void* d_out, uint64_t n, uint32_t elem_size, int kind,
  * No counterpart in the reference. */
 csx_status csx_copy(void* d_dst, const void* d_src, uint64_t bytes, hipStream_t stream);
 
+/* What HBM sustains for plain access mixes in the writeout's access shape:
+ * one block of 256 threads per 256 * vectors_per_thread 16-byte vectors (a
+ * grid covering the buffer, no loop), non-temporal loads and stores, blocks
+ * in runs of xcd_run per XCD (0: plain order).  kind CSX_PROBE_READ reads
+ * d_src (d_dst: 16 bytes that are written only on a hash match);
+ * CSX_PROBE_WRITE writes d_dst; CSX_PROBE_COPY copies; CSX_PROBE_MIX reads
+ * `bytes` of d_src and writes bytes / 2 to d_dst (output vector o = input
+ * vectors 2o xor 2o+1): the byte mix of a 50 % compaction.  bytes a multiple
+ * of 32, pointers 16-byte aligned, vectors_per_thread 1 or 4.  The roofline
+ * context the benchmark reports beside the compaction; no counterpart in the
+ * reference. */
+#define CSX_PROBE_READ 0
+#define CSX_PROBE_WRITE 1
+#define CSX_PROBE_COPY 2
+#define CSX_PROBE_MIX 3
+csx_status csx_stream_probe(int kind, void* d_dst, const void* d_src, uint64_t bytes, uint32_t vectors_per_thread, uint32_t xcd_run,
+                            hipStream_t stream);
+
 /* ---------------------------------------------------------------- profiling
  * Kernel timing for benchmark drivers: after csx_profile_begin(), every
  * csx_compact / csx_writeout call on this host thread records a HIP event
