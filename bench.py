@@ -576,21 +576,55 @@ def cpu_baseline(data_dev, n, budget_s):
 
 # ---------------------------------------------------------------- sweeps
 # `python bench.py --sweep NAME [--reps R] [--log2n L] [--json FILE]`: one CSV
-# line per point on stdout, each time the median of R HIP-event timings of
-# the whole call (or of one phase), every count checked.
+# line per point on stdout, each the per-call time of the whole call (or of
+# one phase), every count checked.
 
 def median_ms(fn, reps):
+    """Per-call time as the device sees it in a stream of calls: R calls
+    enqueued back to back between two HIP events, three rounds after a
+    warm-up, the median round divided by R (the way the autotuner times its
+    candidates; one call alone between events would also time the host's
+    launch path)."""
     fn()
     times = []
-    for _ in range(reps):
+    for _ in range(3):
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record()
-        fn()
+        for _ in range(reps):
+            fn()
         b.record()
         b.synchronize()
-        times.append(a.elapsed_time(b))
+        times.append(a.elapsed_time(b) / reps)
     times.sort()
-    return times[len(times) // 2]
+    return times[1]
+
+
+def graph_ms(fn, reps):
+    """Per-call device time without the host: R calls captured in one graph,
+    replayed in three rounds after a warm-up, the median round divided by R.
+    `fn(stream)` enqueues one call on the given stream."""
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        fn(s)
+        s.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            for _ in range(reps):
+                fn(s)
+    torch.cuda.synchronize()
+    g.replay()
+    times = []
+    for _ in range(3):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        g.replay()
+        b.record()
+        b.synchronize()
+        times.append(a.elapsed_time(b) / reps)
+    times.sort()
+    del g
+    return times[1]
 
 
 def _count_check(count, k, what):
@@ -777,12 +811,13 @@ def sweep_chunks(args):
 def sweep_sizes(args):
     """Call time against column length: u32 at Bernoulli 0.5 from 2^12 to
     2^26 elements, every pipeline (chunk 1024, default tile) and the tuned
-    call; at these sizes launches, not bytes, set the time."""
+    call, each timed as R calls replayed from one captured graph (device time
+    per call, no host launch path); at these sizes launches, not bytes, set
+    the time."""
     count = torch.empty(1, dtype=torch.int64, device="cuda")
     rows = []
     pipes = (("three_phase", csx.PIPELINE_THREE_PHASE), ("two_pass", csx.PIPELINE_TWO_PASS), ("fused", csx.PIPELINE_FUSED),
-             ("single", getattr(csx, "PIPELINE_SINGLE", None)))
-    pipes = [(a, b) for a, b in pipes if b is not None]
+             ("single", csx.PIPELINE_SINGLE))
     print("log2n;" + ";".join(f"{p}_us" for p, _ in pipes) + ";tuned_us;tuned_config", flush=True)
     for ln in range(12, 27, 2):
         n = 1 << ln
@@ -793,10 +828,14 @@ def sweep_sizes(args):
         _, k = csx.compact(data, mask, n=n)
         row = {"log2n": ln, "selected": k}
         for pname, pl in pipes:
-            t = median_ms(lambda: csx.compact(data, mask, n=n, pipeline=pl, out=out, count=count, workspace=ws, host_count=False), args.reps)
+            if pl == csx.PIPELINE_SINGLE and n > csx.SINGLE_MAX_ELEMENTS:
+                row[f"{pname}_us"] = None
+                continue
+            t = graph_ms(lambda st: csx.compact(data, mask, n=n, pipeline=pl, out=out, count=count, workspace=ws, host_count=False, stream=st),
+                         args.reps)
             _count_check(count, k, f"2^{ln} {pname}")
             row[f"{pname}_us"] = round(t * 1000, 2)
-        t = median_ms(lambda: csx.compact_auto(data, mask, 0.5, n=n, out=out, count=count, workspace=ws, host_count=False), args.reps)
+        t = graph_ms(lambda st: csx.compact_auto(data, mask, 0.5, n=n, out=out, count=count, workspace=ws, host_count=False, stream=st), args.reps)
         _count_check(count, k, f"2^{ln} tuned")
         row["tuned_us"] = round(t * 1000, 2)
         row["tuned_config"] = csx.tune_lookup(4, n, 0.5).as_dict()

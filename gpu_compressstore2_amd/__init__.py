@@ -36,6 +36,8 @@ PIPELINE_FUSED = 0
 PIPELINE_THREE_PHASE = 1
 PIPELINE_TWO_PASS = 2
 PIPELINE_SPARSE = 3
+PIPELINE_SINGLE = 4
+SINGLE_MAX_ELEMENTS = 1 << 24
 
 MAX_ELEMENTS = 1 << 40
 

@@ -221,6 +221,11 @@ hipError_t dispatch_sparse(uint32_t w, int order, bool mask_aligned, const csx::
 
 bool launch_fits(uint64_t tiles) { return tiles * 256ull < (1ull << 32); }
 
+bool known_pipeline(int p)
+{
+    return p == CSX_PIPELINE_FUSED || p == CSX_PIPELINE_THREE_PHASE || p == CSX_PIPELINE_TWO_PASS || p == CSX_PIPELINE_SPARSE || p == CSX_PIPELINE_SINGLE;
+}
+
 // ---- dominant-kernel timing (csx_profile_begin / csx_profile_end) ----
 struct Profiler {
     bool on = false;
@@ -322,6 +327,8 @@ csx_config lookup_config(uint32_t w, uint64_t n, double sel)
     const TuneCell* best = nullptr;
     for (const TuneCell& c : t.cells) {
         if (c.w != w) continue;
+        // the nearest cell may be smaller: never past the single launch's cap
+        if (c.cfg.pipeline == CSX_PIPELINE_SINGLE && n > CSX_SINGLE_MAX_ELEMENTS) continue;
         if (!best) {
             best = &c;
             continue;
@@ -621,8 +628,9 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
     auto timed = [&](hipStream_t st, auto&& fn) -> hipError_t { return profile ? ::timed(st, fn) : fn(); };
     csx_status s;
     if ((s = check_width(elem_size)) || (s = check_chunk(chunk_len)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
-    if (pipeline != CSX_PIPELINE_FUSED && pipeline != CSX_PIPELINE_THREE_PHASE && pipeline != CSX_PIPELINE_TWO_PASS && pipeline != CSX_PIPELINE_SPARSE)
-        return fail(CSX_ERR_INVALID, "unknown pipeline %d", pipeline);
+    if (!known_pipeline(pipeline)) return fail(CSX_ERR_INVALID, "unknown pipeline %d", pipeline);
+    if (pipeline == CSX_PIPELINE_SINGLE && n > CSX_SINGLE_MAX_ELEMENTS)
+        return fail(CSX_ERR_UNSUPPORTED, "the single-launch pipeline takes at most %llu elements", (unsigned long long)CSX_SINGLE_MAX_ELEMENTS);
     if ((s = check_vpt(vpt, elem_size))) return s;
     if (pipeline != CSX_PIPELINE_THREE_PHASE && vpt != 8) return fail(CSX_ERR_INVALID, "only the three-phase writeout takes a tile size other than 8");
     const CompactPlan p = plan_compact(n, elem_size, chunk_len);
@@ -675,6 +683,17 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
         a.count_out = count_dst;
         CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeFused, true, bit_order, aligned, a, stream); }),
                 "csx_compact fused");
+    }
+    else if (pipeline == CSX_PIPELINE_SINGLE) {
+        csx::TileArgs a{};
+        a.in = in;
+        a.mask = d_mask;
+        a.out = out;
+        a.n = n;
+        a.count_out = count_dst;
+        a.hdr = hdr;
+        CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeSingle, true, bit_order, aligned, a, stream); }),
+                "csx_compact single launch");
     }
     else if (pipeline == CSX_PIPELINE_TWO_PASS) {
         auto tcounts = reinterpret_cast<uint32_t*>(base + p.tcounts_off);
@@ -1345,6 +1364,11 @@ struct EventPair {
 };
 
 // median over reps of the time of fn() on stream s, after one warm-up run
+// Per-call time of `fn` as the device sees it in a stream of calls: `reps`
+// calls enqueued back to back between two events, in three rounds after a
+// warm-up; the median round's time divided by reps.  (One call bracketed by
+// events alone would also time the host's launch path, which at mid sizes
+// outweighs the kernels and would hide their differences.)
 template <typename F> csx_status time_median(F&& fn, uint32_t reps, hipStream_t s, double* ms)
 {
     EventPair ev;
@@ -1352,18 +1376,20 @@ template <typename F> csx_status time_median(F&& fn, uint32_t reps, hipStream_t 
     CSX_HIP(hipEventCreate(&ev.b), "tune event");
     csx_status st;
     if ((st = fn())) return st;
+    if (reps == 0) reps = 1;
     std::vector<double> t;
-    for (uint32_t r = 0; r < reps; r++) {
+    for (int round = 0; round < 3; round++) {
         CSX_HIP(hipEventRecord(ev.a, s), "tune record");
-        if ((st = fn())) return st;
+        for (uint32_t r = 0; r < reps; r++)
+            if ((st = fn())) return st;
         CSX_HIP(hipEventRecord(ev.b, s), "tune record");
         CSX_HIP(hipEventSynchronize(ev.b), "tune sync");
         float f = 0.f;
         CSX_HIP(hipEventElapsedTime(&f, ev.a, ev.b), "tune elapsed");
-        t.push_back(f);
+        t.push_back(f / reps);
     }
     std::sort(t.begin(), t.end());
-    *ms = t.empty() ? 0.0 : t[t.size() / 2];
+    *ms = t[t.size() / 2];
     return CSX_OK;
 }
 
@@ -1381,6 +1407,7 @@ csx_status tune_candidates(const void* in, const uint8_t* mask, void* out, uint6
     cand.push_back(csx_config{CSX_PIPELINE_SPARSE, 1024, 8, 0});
     cand.push_back(csx_config{CSX_PIPELINE_SPARSE, 4096, 8, 0});
     cand.push_back(csx_config{CSX_PIPELINE_FUSED, 1024, 8, 0});
+    if (n <= CSX_SINGLE_MAX_ELEMENTS) cand.push_back(csx_config{CSX_PIPELINE_SINGLE, 1024, 8, 0});
 
     auto hdr = static_cast<csx::WsHeader*>(ws);
     const uint64_t* count_src = d_count ? d_count : &hdr->count;
@@ -1534,8 +1561,7 @@ csx_status csx_tune_load(const char* path)
             end >= 0 && row[end] == '\0') {
             if (check_width(c.w) || check_chunk(c.cfg.chunk_len) || check_vpt(c.cfg.vpt, c.w) || c.log2n > 40 || !(c.sel >= 0 && c.sel <= 1) ||
                 !(c.ms >= 0 && std::isfinite(c.ms)) ||
-                (c.cfg.pipeline != CSX_PIPELINE_THREE_PHASE && c.cfg.pipeline != CSX_PIPELINE_TWO_PASS && c.cfg.pipeline != CSX_PIPELINE_FUSED &&
-                 c.cfg.pipeline != CSX_PIPELINE_SPARSE) ||
+                !known_pipeline(c.cfg.pipeline) || (c.cfg.pipeline == CSX_PIPELINE_SINGLE && c.log2n > 24) ||
                 (c.cfg.pipeline != CSX_PIPELINE_THREE_PHASE && c.cfg.vpt != 8)) {
                 st = fail(CSX_ERR_INVALID, "%s:%u: configuration out of range", path, lineno);
                 break;

@@ -404,6 +404,56 @@ template <uint32_t TE> __device__ __forceinline__ uint64_t pattern_offset(const 
     return q * a.pat_pop + (uint64_t)__builtin_popcountll(a.pat_bits & ((1ull << r) - 1ull));
 }
 
+// Selected elements before element tbase (a multiple of 128, below n): the
+// popcount of mask bytes [0, tbase / 8), read as 16-byte vectors (a byte's
+// popcount does not depend on its bit order).  prefix_start issues the first
+// kPrefixLoads vectors per thread (64 KiB per tile) -- at the start of the
+// tile, so they are in flight with its own mask and data; prefix_finish
+// counts them and the rest, kPrefixLoads per thread at a time, and returns
+// the block's sum to every thread.  Every thread of the block calls both;
+// prefix_finish contains __syncthreads.
+constexpr int kPrefixLoads = 16;
+struct PrefixBatch {
+    u32x4 x[kPrefixLoads];
+};
+template <typename C, bool ALIGNED> __device__ __forceinline__ void prefix_start(const TileArgs& a, uint64_t tbase, PrefixBatch& pb)
+{
+    const uint64_t nv = tbase / 128;
+    const uint64_t nbytes = (a.n + 7) / 8;
+#pragma unroll
+    for (int j = 0; j < kPrefixLoads; j++) {
+        const uint64_t v = (uint64_t)j * C::BLOCK + threadIdx.x;
+        pb.x[j] = v < nv ? load_mask16<ALIGNED>(a.mask, v * 16, nbytes) : u32x4{0, 0, 0, 0};
+    }
+}
+template <typename C, bool ALIGNED>
+__device__ __forceinline__ uint64_t prefix_finish(const TileArgs& a, uint64_t tbase, const PrefixBatch& pb, TileShared<C>& sh)
+{
+    const uint32_t t = threadIdx.x;
+    const uint64_t nv = tbase / 128;
+    const uint64_t nbytes = (a.n + 7) / 8;
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < kPrefixLoads; j++) c += popc32(pb.x[j].x) + popc32(pb.x[j].y) + popc32(pb.x[j].z) + popc32(pb.x[j].w);
+    for (uint64_t v0 = (uint64_t)kPrefixLoads * C::BLOCK; v0 < nv; v0 += (uint64_t)kPrefixLoads * C::BLOCK) {
+        u32x4 x[kPrefixLoads];
+#pragma unroll
+        for (int j = 0; j < kPrefixLoads; j++) {
+            const uint64_t v = v0 + (uint64_t)j * C::BLOCK + t;
+            x[j] = v < nv ? load_mask16<ALIGNED>(a.mask, v * 16, nbytes) : u32x4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int j = 0; j < kPrefixLoads; j++) c += popc32(x[j].x) + popc32(x[j].y) + popc32(x[j].z) + popc32(x[j].w);
+    }
+    const uint32_t s = wave_sum_u32(c);
+    if (lane_id() == 0) sh.wtot[t / kWave] = s;
+    __syncthreads();
+    uint64_t all = 0;
+#pragma unroll
+    for (int k = 0; k < C::BLOCK / kWave; k++) all += sh.wtot[k];
+    return all;
+}
+
 // One tile: every thread of the block calls this with the same `tile`.
 // Contains __syncthreads; LDS in `sh` may be reused once all threads return
 // and pass a barrier.
@@ -446,6 +496,11 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
 #pragma unroll
         for (int k = 0; k < 8; k++) psum += acc[k];
     }
+
+    // single launch: the first part of the mask before the tile, in flight
+    // with the tile's own work
+    PrefixBatch pfx;
+    if constexpr (MODE == kModeSingle) prefix_start<C, ALIGNED>(a, tbase, pfx);
 
     // three-phase writeout with two-level offsets: the tile's block prefix
     // and its unit's offset, requested ahead of the mask
@@ -552,6 +607,12 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
             if (MODE == kModeTwoPass && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = prefix_total;
             if (MODE == kModeGiven && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = a.given_off;
             if (MODE == kModePattern && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = pattern_offset<C::TE>(a, tile);
+            if constexpr (MODE == kModeSingle) {
+                if (tile == ntiles - 1 && a.count_out) {
+                    const uint64_t pre = prefix_finish<C, ALIGNED>(a, tbase, pfx, sh);
+                    if (t == 0) *a.count_out = pre;
+                }
+            }
             return;
         }
     }
@@ -631,6 +692,15 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
                 sh.off = pre;
                 if (tile == ntiles - 1 && a.count_out) *a.count_out = pre + tile_count;
             }
+        }
+        __syncthreads();
+    }
+    else if constexpr (MODE == kModeSingle) {
+        // finished once this tile's data loads are in flight
+        const uint64_t pre = prefix_finish<C, ALIGNED>(a, tbase, pfx, sh);
+        if (t == 0) {
+            sh.off = pre;
+            if (tile == ntiles - 1 && a.count_out) *a.count_out = pre + tile_count;
         }
         __syncthreads();
     }
@@ -717,9 +787,12 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a)
         __syncthreads();
         tile = sh.tile;
     }
-    else if constexpr (MODE == kModeWriteout || MODE == kModePattern) {
+    else if constexpr (MODE == kModeWriteout || MODE == kModePattern || MODE == kModeSingle) {
         // the grid is exactly one block per tile
         tile = xcd_runs(blockIdx.x, gridDim.x);
+        // the single launch clears the workspace error word (nothing of it sets
+        // the word, and the workspace error query answers for this call)
+        if (MODE == kModeSingle && a.hdr && blockIdx.x == 0 && threadIdx.x == 0) a.hdr->error = 0;
     }
     else {
         tile = blockIdx.x;

@@ -30,6 +30,9 @@ static void launch_tile_impl(int mode, bool staged, int vpt, const TileArgs& a, 
     else if (mode == kModeFused) {
         launch_k(k_tile<W, ORDER, ALIGNED, kModeFused, true>, dim3(grid), dim3(256), s, a);
     }
+    else if (mode == kModeSingle) {
+        launch_k(k_tile<W, ORDER, ALIGNED, kModeSingle, true>, dim3(grid), dim3(256), s, a);
+    }
     else if (mode == kModeGiven) {
         // one tile at a.given_off (inputs of at most one tile: the whole call)
         launch_k(k_tile<W, ORDER, ALIGNED, kModeGiven, true>, dim3(grid), dim3(256), s, a);

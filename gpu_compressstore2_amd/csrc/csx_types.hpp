@@ -41,7 +41,9 @@ inline uint64_t popb_blocks(uint64_t n) { return (n + (1ull << popb_log2(n)) - 1
 // kModeGiven: the tile's output offset is supplied in TileArgs::given_off
 // kModePattern: no mask in memory; mask words and the tile's offset follow
 // from a periodic pattern (TileArgs::pat_*)
-enum TileMode { kModeWriteout = 0, kModeFused = 1, kModeTwoPass = 2, kModeGiven = 3, kModePattern = 4 };
+// kModeSingle: the tile's offset is the popcount of the whole mask before
+// it, counted by the tile itself (one launch, no other kernel's results)
+enum TileMode { kModeWriteout = 0, kModeFused = 1, kModeTwoPass = 2, kModeGiven = 3, kModePattern = 4, kModeSingle = 5 };
 
 // Arguments of one tile-engine launch.
 struct TileArgs {

@@ -76,8 +76,13 @@ typedef enum {
                                      the fastest form at low and medium selectivity (default of the bindings) */
     CSX_PIPELINE_TWO_PASS = 2,    /* pass 1 counts every tile and group of tiles from the mask; pass 2 is the staged
                                      writeout, each tile reducing its own offset from those counts */
-    CSX_PIPELINE_SPARSE = 3       /* csx_popc -> csx_scan -> csx_writeout (SPARSE): for low selectivity */
+    CSX_PIPELINE_SPARSE = 3,      /* csx_popc -> csx_scan -> csx_writeout (SPARSE): for low selectivity */
+    CSX_PIPELINE_SINGLE = 4       /* one launch: each tile counts the whole mask before it for its own output offset,
+                                     so no other kernel runs first; quadratic in n, for columns of at most
+                                     CSX_SINGLE_MAX_ELEMENTS (mid-size calls, where launches, not bytes, cost time).
+                                     No counterpart in the reference. */
 } csx_pipeline;
+#define CSX_SINGLE_MAX_ELEMENTS (UINT64_C(1) << 24)
 
 /* Largest element count accepted (offsets are 62-bit in the look-back status words). */
 #define CSX_MAX_ELEMENTS (UINT64_C(1) << 40)
@@ -168,7 +173,8 @@ csx_status csx_writeout_w8(const uint64_t* d_in, const uint8_t* d_mask, const ui
  * (n is always enough); nothing at or past d_out[k] is written.  Not in
  * place: d_out's n elements must not overlap d_in or d_mask (checked, as
  * for every entry point below that takes both; CSX_ERR_INVALID).
- * chunk_len only matters for CSX_PIPELINE_THREE_PHASE and CSX_PIPELINE_SPARSE.
+ * chunk_len only matters for CSX_PIPELINE_THREE_PHASE and CSX_PIPELINE_SPARSE,
+ * whose popcount counts units of min(chunk_len, writeout tile) elements.
  * CSX_PIPELINE_THREE_PHASE is the default of the language bindings.
  * Replaces the benchN compositions (src/benchmarks.cuh:153-440) -- bench10
  * is CSX_PIPELINE_THREE_PHASE with the staged writeout. */
@@ -305,8 +311,10 @@ csx_status csx_compact_auto(const void* d_in, const uint8_t* d_mask, void* d_out
                             uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream);
 /* Autotuner for one cell: generates n elements and a Bernoulli(selectivity)
  * mask on the device, times every candidate (THREE_PHASE x chunk {256, 1024,
- * 4096} x vpt {2, 4, 8, 16}, TWO_PASS, FUSED) with HIP events -- the median
- * of `reps` runs after one warm-up -- checks that all give the same count,
+ * 4096} x vpt {2, 4, 8, 16}, TWO_PASS, FUSED, SPARSE, and SINGLE up to its
+ * cap) with HIP events -- `reps` calls back to back between two events, three
+ * rounds after one warm-up, the median round's per-call time -- checks that
+ * all give the same count,
  * records the fastest in the table and returns it.  Allocates and frees its
  * own device buffers and synchronises the stream: an offline step. */
 csx_status csx_autotune(uint32_t elem_size, uint64_t n, double selectivity, uint32_t reps, csx_config* best, double* best_ms, hipStream_t stream);
@@ -318,7 +326,7 @@ csx_status csx_autotune(uint32_t elem_size, uint64_t n, double selectivity, uint
  * is measured on Bernoulli masks; on masks whose selected elements come in
  * runs (sorted or correlated columns) a larger tile can be faster at the
  * same selectivity, and tuning on such a column records it.  Runs the call
- * reps+1 times per candidate (d_out and the workspace are rewritten each
+ * 3 reps + 1 times per candidate (d_out and the workspace are rewritten each
  * time) and synchronises the stream: an offline step.  No counterpart in the
  * reference. */
 csx_status csx_autotune_inputs(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, int bit_order, uint32_t reps,
@@ -326,7 +334,7 @@ csx_status csx_autotune_inputs(const void* d_in, const uint8_t* d_mask, void* d_
 /* The batched form tuned on the caller's own table (vpt {2, 4, 8} x
  * count_blocks {1024, 2048, 4096, 8192}); the result is recorded for
  * elem_size and used by csx_compact_batch from then on.  Runs the batch
- * reps+1 times per candidate (outputs and counts are rewritten each time). */
+ * 3 reps + 1 times per candidate (outputs and counts are rewritten each time). */
 csx_status csx_autotune_batch(const csx_segment* d_segments, uint32_t nseg, uint64_t max_total_elements, uint32_t elem_size, int bit_order,
                               uint64_t* d_counts, void* d_ws, size_t ws_bytes, uint32_t reps, csx_config* best, double* best_ms,
                               hipStream_t stream);

@@ -109,13 +109,22 @@ def test_null_data_pointers_rejected():
 def test_unknown_pipeline_rejected():
     need = csx.workspace_size(100, 4, 1024)
     assert csx.lib().csx_compact(None, None, None, 100, 4, 1024, 0, 9, None, None, ctypes.c_void_p(4096), need, None) == 1
-    # the highest known pipeline (sparse) passes the pipeline check and stops at the null pointers;
+    # the highest known pipeline (the single launch) passes the pipeline check and stops at the null pointers;
     # one past it is unknown
     L = csx.lib()
-    assert L.csx_compact(None, None, None, 100, 4, 1024, 0, csx.PIPELINE_SPARSE, None, None, ctypes.c_void_p(4096), need, None) == 1
+    assert L.csx_compact(None, None, None, 100, 4, 1024, 0, csx.PIPELINE_SINGLE, None, None, ctypes.c_void_p(4096), need, None) == 1
     assert b"null" in L.csx_last_error()
-    assert L.csx_compact(None, None, None, 100, 4, 1024, 0, csx.PIPELINE_SPARSE + 1, None, None, ctypes.c_void_p(4096), need, None) == 1
+    assert L.csx_compact(None, None, None, 100, 4, 1024, 0, csx.PIPELINE_SINGLE + 1, None, None, ctypes.c_void_p(4096), need, None) == 1
     assert b"unknown pipeline" in L.csx_last_error()
+
+
+def test_single_launch_cap_rejected_before_device_work():
+    L = csx.lib()
+    n = csx.SINGLE_MAX_ELEMENTS + 1
+    need = csx.workspace_size(n, 4, 1024)
+    fake = ctypes.c_void_p(4096)
+    assert L.csx_compact(fake, fake, ctypes.c_void_p(1 << 44), n, 4, 1024, 0, csx.PIPELINE_SINGLE, None, None, fake, need, None) == 4
+    assert b"single-launch" in L.csx_last_error()
 
 
 def test_in_place_calls_rejected_before_device_work():
@@ -162,12 +171,28 @@ def test_tuning_table_accepts_sparse_and_rejects_bad_rows(tmp_path):
         c = csx.tune_lookup(4, 1 << 20, 0.01)
         assert (c.pipeline, c.chunk_len, c.vpt) == (csx.PIPELINE_SPARSE, 4096, 8)
         for bad in ("compact 4 20 0.01 3 4096 2 0.01\n",  # sparse takes no tile size but 8
-                    "compact 4 20 0.01 4 1024 8 0.01\n",  # unknown pipeline
+                    "compact 4 20 0.01 5 1024 8 0.01\n",  # unknown pipeline
+                    "compact 4 25 0.01 4 1024 8 0.01\n",  # the single launch past its 2^24 cap
                     "compact 4 20 1.5 1 1024 8 0.01\n"):  # selectivity out of range
             f = tmp_path / "bad.txt"
             f.write_text(bad)
             with pytest.raises(csx.CsxError):
                 csx.tune_load(str(f))
+    finally:
+        csx.tune_clear()
+        csx.tune_load(csx.TUNED_TABLE)
+
+
+def test_single_launch_cells_never_serve_columns_past_the_cap(tmp_path):
+    f = tmp_path / "t.txt"
+    f.write_text("compact 4 24 0.500000 4 1024 8 0.005\ncompact 4 28 0.500000 1 1024 2 0.27\n")
+    csx.tune_clear()
+    try:
+        csx.tune_load(str(f))
+        assert csx.tune_lookup(4, 1 << 24, 0.5).pipeline == csx.PIPELINE_SINGLE
+        # 2^25 is nearest to the 2^24 cell, which it may not take
+        c = csx.tune_lookup(4, 1 << 25, 0.5)
+        assert (c.pipeline, c.vpt) == (csx.PIPELINE_THREE_PHASE, 2)
     finally:
         csx.tune_clear()
         csx.tune_load(csx.TUNED_TABLE)

@@ -26,13 +26,14 @@ def capture(fn):
     return g
 
 
-@pytest.mark.parametrize("form", ["three_phase", "two_pass", "fused", "sparse", "tuned"])
+@pytest.mark.parametrize("form", ["three_phase", "two_pass", "fused", "sparse", "single", "tuned"])
 def test_compact_graph_replays(O, form):
     data = torch.zeros(N, dtype=torch.int32, device="cuda")
     mask = torch.zeros((N + 7) // 8, dtype=torch.uint8, device="cuda")
     out = torch.empty(N, dtype=torch.int32, device="cuda")
     count = torch.zeros(1, dtype=torch.int64, device="cuda")
-    pipes = {"three_phase": csx.PIPELINE_THREE_PHASE, "two_pass": csx.PIPELINE_TWO_PASS, "fused": csx.PIPELINE_FUSED, "sparse": csx.PIPELINE_SPARSE}
+    pipes = {"three_phase": csx.PIPELINE_THREE_PHASE, "two_pass": csx.PIPELINE_TWO_PASS, "fused": csx.PIPELINE_FUSED, "sparse": csx.PIPELINE_SPARSE,
+             "single": csx.PIPELINE_SINGLE}
     if form == "tuned":
         ws = csx.alloc_workspace(csx.auto_workspace_size(N, 4))
         g = capture(lambda s: csx.compact_auto(data, mask, 0.4, n=N, out=out, count=count, workspace=ws, stream=s, host_count=False))

@@ -82,7 +82,9 @@ def resolve_n(n, w):
     return {"T-1": T - 1, "T": T, "T+1": T + 1, "3T+17": 3 * T + 17}[n]
 
 
-PIPELINES = [csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE, csx.PIPELINE_TWO_PASS, csx.PIPELINE_SPARSE]
+PIPELINES = [csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE, csx.PIPELINE_TWO_PASS, csx.PIPELINE_SPARSE, csx.PIPELINE_SINGLE]
+# the pipelines without a size cap (the single launch stops at 2^24 elements)
+PIPELINES_ANY_N = [csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE, csx.PIPELINE_TWO_PASS, csx.PIPELINE_SPARSE]
 
 
 @pytest.mark.parametrize("pipeline", PIPELINES)
@@ -272,8 +274,33 @@ def test_two_pass_many_groups(O):
     assert k == ke and (got == expect).all()
 
 
+def test_single_launch_at_its_cap():
+    """The single-launch pipeline at CSX_SINGLE_MAX_ELEMENTS (the last tile
+    counts 2 MiB of mask before it) against torch.nonzero, a ragged size just
+    below, and the first size past the cap refused before any launch."""
+    for n, p in ((csx.SINGLE_MAX_ELEMENTS, 0.5), (csx.SINGLE_MAX_ELEMENTS - 77, 0.02)):
+        print(f"single launch n={n} p={p}", flush=True)
+        g = torch.Generator(device="cuda")
+        g.manual_seed(n)
+        bits = torch.rand(n, device="cuda", generator=g) < p
+        pad = (-n) % 8
+        padded = torch.cat([bits, torch.zeros(pad, dtype=torch.bool, device="cuda")])
+        weights = torch.tensor([128, 64, 32, 16, 8, 4, 2, 1], dtype=torch.uint8, device="cuda")
+        mask = (padded.view(-1, 8).to(torch.uint8) * weights).sum(1, dtype=torch.uint8)
+        data = torch.arange(n, dtype=torch.int32, device="cuda")
+        out, k = csx.compact(data, mask, n=n, pipeline=csx.PIPELINE_SINGLE)
+        expect = torch.nonzero(bits).flatten().to(torch.int32)
+        assert k == expect.numel()
+        assert torch.equal(out[:k], expect)
+    big = csx.SINGLE_MAX_ELEMENTS + 1
+    with pytest.raises(csx.CsxError) as e:
+        csx.compact(torch.empty(big, dtype=torch.uint8, device="cuda"), torch.empty((big + 7) // 8, dtype=torch.uint8, device="cuda"), n=big,
+                    pipeline=csx.PIPELINE_SINGLE)
+    assert e.value.status == 4
+
+
 @pytest.mark.parametrize("p", [0.01, 0.5, 0.99])
-@pytest.mark.parametrize("pipeline", PIPELINES)
+@pytest.mark.parametrize("pipeline", PIPELINES_ANY_N)
 def test_full_size_2p28_index_identity(p, pipeline):
     """2^28 u32 with data[i] = i: the output must be exactly the selected
     indices, checked against torch.nonzero of the unpacked mask."""

@@ -22,6 +22,7 @@ def candidates(w):
     out.append(csx.Config(csx.PIPELINE_FUSED, 1024, 8, 0))
     out.append(csx.Config(csx.PIPELINE_SPARSE, 1024, 8, 0))
     out.append(csx.Config(csx.PIPELINE_SPARSE, 4096, 8, 0))
+    out.append(csx.Config(csx.PIPELINE_SINGLE, 1024, 8, 0))
     return out
 
 
@@ -56,7 +57,7 @@ def test_every_candidate_matches_oracle(O, w):
 
 def test_autotune_then_auto(O):
     cfg, ms = csx.autotune(4, 1 << 22, 0.5, reps=3)
-    assert ms > 0 and cfg.vpt in (2, 4, 8, 16) and cfg.pipeline in (0, 1, 2)
+    assert ms > 0 and cfg.vpt in (2, 4, 8, 16) and cfg.pipeline in (0, 1, 2, 4)
     got = csx.tune_lookup(4, 1 << 22, 0.5)
     assert got.as_dict() == cfg.as_dict()
     n = (1 << 22) + 77
