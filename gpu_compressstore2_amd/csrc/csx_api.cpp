@@ -826,7 +826,8 @@ csx_status csx_pattern_compact(const void* d_in, void* d_out, uint64_t n, uint32
     uint64_t k = 0;
     if ((s = check_width(elem_size)) || (s = check_n(n)) || (s = csx_pattern_count(n, pattern, pattern_length, &k))) return s;
     if (k > 0 && (!d_in || !d_out)) return fail(CSX_ERR_INVALID, "null input or output pointer");
-    if (k > 0 && overlaps(d_out, n * elem_size, d_in, n * elem_size))
+    // the count is known here, and nothing at or past d_out[k] is written
+    if (k > 0 && overlaps(d_out, k * elem_size, d_in, n * elem_size))
         return fail(CSX_ERR_INVALID, "the output overlaps the input (compaction is not in place)");
     if (((uintptr_t)d_in % elem_size) || ((uintptr_t)d_out % elem_size))
         return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");

@@ -133,6 +133,7 @@ namespace {
 struct Pending {
     csx_request req;
     double submit_ms;
+    uint64_t seq;  // submission order
 };
 
 // one in-flight batch: its stream slot, requests and count read-back
@@ -165,6 +166,7 @@ struct csx_server {
     std::deque<Batch> inflight;
     uint32_t next_slot = 0;
     uint64_t launched_batches = 0;
+    uint64_t next_seq = 0;  // submission order, kept when a failed batch is re-queued
 };
 
 extern "C" csx_status csx_server_create(const csx_server_config* cfg, csx_server** out)
@@ -229,7 +231,7 @@ extern "C" csx_status csx_server_submit(csx_server* s, const csx_request* r)
         return csx::set_error(CSX_ERR_INVALID, "server: request of %llu elements exceeds max_batch_elements %llu", (unsigned long long)r->n,
                               (unsigned long long)s->cfg.max_batch_elements);
     if (r->n > 0 && (!r->in || !r->mask || !r->out)) return csx::set_error(CSX_ERR_INVALID, "server: null input, mask or output");
-    s->queue.push_back(Pending{*r, now_ms()});
+    s->queue.push_back(Pending{*r, now_ms(), s->next_seq++});
     return csx::clear_error();
 }
 
@@ -239,7 +241,8 @@ extern "C" uint32_t csx_server_inflight(const csx_server* s) { return s ? (uint3
 // Launch batches from the queue while stream slots are free: each batch takes
 // the oldest queued request's width and every queued request of that width,
 // up to max_batch requests and max_batch_elements elements.  A failed launch
-// puts the batch's requests back at the front of the queue.
+// drains the slot's stream and puts the batch's requests back in submission
+// order.
 static csx_status launch_batch(csx_server* s, Slot& sl, const Batch& b, uint64_t elems)
 {
     hipError_t e = hipSuccess;
@@ -296,7 +299,20 @@ extern "C" csx_status csx_server_step(csx_server* s, uint32_t* launched)
         }
         const csx_status st = launch_batch(s, sl, b, elems);
         if (st != CSX_OK) {
-            s->queue.insert(s->queue.begin(), b.reqs.begin(), b.reqs.end());
+            // Part of the batch may already be queued on the slot's stream (the
+            // table upload reads the pinned h_table), so the slot is drained
+            // before any later step rewrites it.  The requests go back in
+            // submission order: merged by sequence number with the requests
+            // still queued, which stay in order themselves.
+            (void)hipStreamSynchronize(sl.stream);
+            std::deque<Pending> merged;
+            auto q = s->queue.begin();
+            for (const Pending& p : b.reqs) {
+                while (q != s->queue.end() && q->seq < p.seq) merged.push_back(*q++);
+                merged.push_back(p);
+            }
+            merged.insert(merged.end(), q, s->queue.end());
+            s->queue.swap(merged);
             if (launched) *launched = nl;
             return st;
         }

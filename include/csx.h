@@ -249,8 +249,8 @@ uint64_t csx_arena_used(const csx_arena* arena);
  * still on the device; csx_server_poll retires finished batches without
  * blocking and reports, per request, the caller's id, the selected count and
  * the latency from submission to that poll (`capacity` must hold a whole
- * batch: at least max_batch completions).  A step that fails puts the
- * requests of the failed batch back at the front of the queue.  The results
+ * batch: at least max_batch completions).  A step that fails waits for the
+ * failed batch's stream and puts its requests back in submission order.  The results
  * equal csx_compact's on each request alone.  One host thread drives a
  * server; csx_server_destroy waits for in-flight batches.  No counterpart in
  * the reference. */

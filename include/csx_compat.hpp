@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <iomanip>
 #include <iostream>
 #include <vector>
@@ -477,7 +478,33 @@ timings bench_csx_pipeline(intermediate_data* id, int pipeline, T* d_input, uint
     return t;
 }
 
-// src/benchmarks.cuh:442-450
+// Mismatch report of the reference's validation with report_failures
+// (src/kernels/data_generator.cuh:136-147): one line per differing element,
+// "failure: index: I: expected: E, got: G" with E and G the elements' bits
+// in hex, here in index order and at most `limit` lines, then the number of
+// further mismatches.  Host arrays; returns the number of mismatches.
+template <typename T>
+uint64_t report_mismatches(const T* expected, const T* got, uint64_t len, uint64_t limit, FILE* f)
+{
+    uint64_t bad = 0;
+    for (uint64_t i = 0; i < len; i++) {
+        uint64_t e = 0, g = 0;
+        memcpy(&e, &expected[i], sizeof(T));
+        memcpy(&g, &got[i], sizeof(T));
+        if (e == g) continue;
+        if (bad < limit) fprintf(f, "failure: index: %llu: expected: %llx, got: %llx\n", (unsigned long long)i, (unsigned long long)e, (unsigned long long)g);
+        bad++;
+    }
+    if (bad > limit) fprintf(f, "... %llu more mismatching elements\n", (unsigned long long)(bad - limit));
+    return bad;
+}
+
+// Lines of mismatch report per failed validation (the reference prints all).
+constexpr uint64_t kReportLimit = 64;
+
+// src/benchmarks.cuh:442-450: the device counts the mismatches; with
+// report_failures a failed validation copies both arrays back and reports
+// them element by element (report_mismatches).
 template <typename T>
 bool validate(intermediate_data* id, T* d_validation, T* d_output, uint64_t out_length, bool report_failures, uint64_t* failure_count = nullptr)
 {
@@ -485,7 +512,12 @@ bool validate(intermediate_data* id, T* d_validation, T* d_output, uint64_t out_
     check(csx_validate(d_validation, d_output, out_length, sizeof(T), id->d_failure_count, 0), "validate");
     uint64_t fc = 0;
     hip_check(hipMemcpy(&fc, id->d_failure_count, sizeof(fc), hipMemcpyDeviceToHost), "hipMemcpy");
-    if (report_failures && fc) fprintf(stderr, "validation: %llu mismatching elements\n", (unsigned long long)fc);
+    if (report_failures && fc) {
+        std::vector<T> e(out_length), g(out_length);
+        hip_check(hipMemcpy(e.data(), d_validation, out_length * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy");
+        hip_check(hipMemcpy(g.data(), d_output, out_length * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy");
+        report_mismatches(e.data(), g.data(), out_length, kReportLimit, stderr);
+    }
     if (failure_count) *failure_count = fc;
     return fc == 0;
 }

@@ -27,6 +27,34 @@ def test_driver_argument_checks(args, msg):
     assert r.returncode != 0 and msg in r.stderr
 
 
+def test_mismatch_report_format(tmp_path):
+    """The driver's -r report (csx_compat::report_mismatches, the reference's
+    per-failure lines of src/kernels/data_generator.cuh:136-147): index,
+    expected and got bits in hex, in index order, capped with a remainder
+    line; checked on host arrays with planted mismatches (no GPU)."""
+    exe = tmp_path / "report"
+    src = os.path.join(ROOT, "tests", "cpp", "report_mismatches.cpp")
+    b = subprocess.run(["g++", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"), src, "-o",
+                        str(exe)], capture_output=True, text=True)
+    assert b.returncode == 0, b.stderr[-2000:]
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines()
+    e8 = [(0x0102030405060708 * (i + 1)) & 0xFF for i in range(1000)]
+    e32 = [(0x0102030405060708 * (i + 1)) & 0xFFFFFFFF for i in range(1000)]
+    expect = [
+        f"failure: index: 0: expected: {e8[0]:x}, got: {e8[0] ^ 0x5A:x}",
+        f"failure: index: 999: expected: {e8[999]:x}, got: {e8[999] ^ 0x5A:x}",
+        "count 2",
+        f"failure: index: 5: expected: {e32[5]:x}, got: {e32[5] ^ 0x5A:x}",
+        f"failure: index: 17: expected: {e32[17]:x}, got: {e32[17] ^ 0x5A:x}",
+        "... 2 more mismatching elements",
+        "count 4",
+        "count 0",
+    ]
+    assert out[:len(expect)] == expect
+    # a double's bits: expected and got differ only in the low byte
+    assert out[len(expect)].startswith("failure: index: 3: expected: ") and out[-1] == "count 1"
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("args", [
     ["-l", "100003", "-i", "2", "-k", "256", "-c", "1024"],                # default: seed-42 pattern mask, p = 0.5, L = 8

@@ -80,6 +80,51 @@ def test_batch_every_tile_size_and_count_grid(O, w):
                 check(cols, outs, counts, w)
 
 
+@pytest.mark.parametrize("w", [1, 2, 4, 8])
+def test_batch_unaligned_segments(O, w):
+    """Columns sliced at offsets that break 16-byte alignment: input and
+    output off by whole elements, masks off by 1..15 bytes, every third
+    column aligned, so aligned and unaligned columns share the packed wave
+    loads of the count kernel (2- and 4-vector tiles) and the write kernel
+    runs its unaligned tile body on tiles after a column's first."""
+    T2 = TILE[w] // 4  # elements of the 2-vector tile
+    rng = np.random.default_rng(70 + w)
+    lengths = [int(x) for x in rng.integers(1, 3 * T2, 40)] + [T2, 2 * T2 + 1, 5, 9 * T2 + 3]
+    per16 = 16 // w
+    for order in (oracle.MSB_FIRST, oracle.LSB_FIRST):
+        cols, ins, masks, outs = [], [], [], []
+        for i, n in enumerate(lengths):
+            aligned = i % 3 == 0
+            eo = 0 if aligned or per16 == 1 else int(rng.integers(1, per16))
+            mo = 0 if aligned else int(rng.integers(1, 16))
+            oo = 0 if aligned or per16 == 1 else int(rng.integers(1, per16))
+            data = O.fill_values(n, NP[w], seed=900 + 31 * i + w)
+            mask = O.mask_bernoulli(n, float(rng.uniform(0.05, 0.95)), seed=901 + 31 * i + w, bit_order=order)
+            expect, k = O.compact(data, mask, n, order)
+            cols.append((data, mask, expect, k))
+            ibuf = torch.zeros(n + per16 + 1, dtype=dev(data[:1]).dtype, device="cuda")
+            ibuf[eo:eo + n] = dev(data)
+            mbuf = torch.zeros(mask.size + 16, dtype=torch.uint8, device="cuda")
+            mbuf[mo:mo + mask.size] = dev(mask)
+            obuf = torch.full((n + per16 + 4,), -1, dtype=ibuf.dtype, device="cuda")
+            ins.append(ibuf[eo:eo + n])
+            masks.append(mbuf[mo:mo + mask.size])
+            outs.append(obuf[oo:])
+        table = csx.segment_table(ins, masks, outs, lengths)
+        total = sum(lengths)
+        ws = csx.alloc_workspace(csx.batch_workspace_size(len(lengths), total, w))
+        counts = torch.zeros(len(lengths), dtype=torch.int64, device="cuda")
+        for vpt in (2, 4, 8):
+            for cb in (7, 4096):
+                print(f"batch unaligned w={w} order={order} vpt={vpt} count_blocks={cb}", flush=True)
+                for o in outs:
+                    o.fill_(-1)
+                csx.compact_batch_config(table, w, total, csx.Config(csx.PIPELINE_THREE_PHASE, 0, vpt, cb), counts=counts, workspace=ws,
+                                         bit_order=order)
+                torch.cuda.synchronize()
+                check(cols, outs, counts, w)
+
+
 def test_batch_many_log_uniform_columns(O):
     rng = np.random.default_rng(4)
     lengths = [int(x) for x in np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 16), 300))]

@@ -370,6 +370,17 @@ def test_pattern_compact(O, w):
             got = host(d_out, NP[w])
             assert k == ke and int(d_cnt.item()) == ke
             assert (got[:k] == expect).all() and (got[k:] == canary[k:]).all()
+    # an output of exactly the k selected elements right before the input in
+    # one allocation (an arena's neighbouring ranges): accepted, and nothing
+    # of the input beyond is touched
+    pat, L, n = 0xB5000000, 8, 100003
+    data = O.fill_values(n, NP[w], seed=17)
+    mask, _ = O.mask_pattern((n + 7) // 8, pat, L)
+    expect, ke = O.compact(data, mask, n)
+    buf = dev(np.concatenate([np.zeros(ke, NP[w]), data]))
+    out, k = csx.pattern_compact(buf[ke:], pat, L, n=n, out=buf[:ke])
+    got = host(buf, NP[w])
+    assert k == ke and (got[:k] == expect).all() and (got[ke:] == data).all()
     # the reference driver's own pattern (src/main.cu:54-64) at p = 0.5, L = 8
     pat = O.pattern_from_selectivity(8, 0.5)
     n = 300000

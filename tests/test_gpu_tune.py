@@ -28,9 +28,12 @@ def candidates(w):
 
 @pytest.fixture(autouse=True)
 def clean_table():
+    """An empty table for the test; the shipped table again afterwards, so
+    that later tests run the tuned configurations whatever the order."""
     csx.tune_clear()
     yield
     csx.tune_clear()
+    csx.tune_load(csx.TUNED_TABLE)
 
 
 @pytest.mark.parametrize("w", [1, 2, 4, 8])

@@ -9,9 +9,12 @@ import gpu_compressstore2_amd as csx
 
 @pytest.fixture(autouse=True)
 def clean_table():
+    """An empty table for the test; the shipped table again afterwards, so
+    that later tests run the tuned configurations whatever the order."""
     csx.tune_clear()
     yield
     csx.tune_clear()
+    csx.tune_load(csx.TUNED_TABLE)
 
 
 def test_default_without_measurements():
