@@ -106,10 +106,12 @@ def time_steps(fn, steps, warmup):
     return a.elapsed_time(b)
 
 
-def bench_batch(steps, warmup, seed=2024, nseg=4096):
+def bench_batch(steps, warmup, seed=2024, nseg=4096, sparse_share=0.0, batched_only=False):
     """4096 u32 columns, lengths log-uniform in [2^10, 2^24], per-column
-    Bernoulli selectivity uniform in [0.01, 0.99]: one batched call against
-    one csx_compact call per column (both eager, and both graph-replayed)."""
+    Bernoulli selectivity uniform in [0.01, 0.99] (with sparse_share, that
+    share of the columns at selectivity uniform in [0.001, 0.01] instead):
+    one batched call against one csx_compact call per column (both eager, and
+    both graph-replayed; batched_only: the batched call alone)."""
     import ctypes
 
     import numpy as np
@@ -117,6 +119,9 @@ def bench_batch(steps, warmup, seed=2024, nseg=4096):
     rng = np.random.default_rng(seed)
     lengths = np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 24), nseg)).astype(np.int64)
     sels = rng.uniform(0.01, 0.99, nseg)
+    if sparse_share > 0:
+        pick = rng.random(nseg) < sparse_share
+        sels[pick] = rng.uniform(0.001, 0.01, int(pick.sum()))
     starts = np.zeros(nseg, np.int64)
     mstarts = np.zeros(nseg, np.int64)
     pos = mpos = 0
@@ -158,6 +163,21 @@ def bench_batch(steps, warmup, seed=2024, nseg=4096):
     if not np.array_equal(counts.cpu().numpy(), ks):
         print("[bench] MISMATCH batched counts with the tuned configuration", file=sys.stderr)
         sys.exit(1)
+    if batched_only:
+        res["sparse_share"] = sparse_share
+        csx.compact_batch(table, 4, total, counts=counts, workspace=ws)
+        torch.cuda.synchronize()
+        for i in (0, nseg // 2, int(np.argmax(lengths)), int(np.argmin(sels))):
+            n = int(lengths[i])
+            expect = torch.masked_select(ins[i], unpack_msb_first(ms[i], n))
+            k = int(counts[i].item())
+            if k != expect.numel() or not torch.equal(outs[i][:k], expect):
+                print(f"[bench] batch MISMATCH in column {i}", file=sys.stderr)
+                sys.exit(1)
+        print(f"[bench] batch (sparse share {sparse_share}): {res}", file=sys.stderr, flush=True)
+        del data, out, masks, table, ws
+        torch.cuda.empty_cache()
+        return res
 
     # one csx_compact per column, through the C ABI with prebuilt arguments
     L = csx.lib()
@@ -390,6 +410,7 @@ def main():
             del datas
             torch.cuda.empty_cache()
             sweep["batch_4096_columns"] = bench_batch(args.steps, args.warmup)
+            sweep["batch_4096_columns_half_sparse"] = bench_batch(args.steps, args.warmup, sparse_share=0.5, batched_only=True)
             datas = [make_data(n, 100 + r) for r in range(ROTATE)]
 
         sweep["predicate_f32_gt_0.5"] = bench_predicate(datas, n, K, W, out, count)

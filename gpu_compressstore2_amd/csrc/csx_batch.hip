@@ -12,7 +12,10 @@
 //   K2 scan    the look-back scan over all tile counts (length from device)
 //   K3 write   one workgroup per (segment, tile): the tile engine with the
 //              tile's offset made local to its segment; the last tile of a
-//              segment writes the segment's selected count
+//              segment writes the segment's selected count; empty tiles and
+//              tiles of at most 1/64 density leave at once
+//   K4 sparse  persistent grid over the tiles of at most 1/64 density: one
+//              wave per tile, selected elements gathered without LDS
 // The reference's closest pattern is its two-stream split with a carry
 // (src/benchmarks.cuh:195-293); here the split is by column, needing no carry.
 #include "csx_internal.hpp"
@@ -256,9 +259,15 @@ __global__ __launch_bounds__(256) void k_batch_counts(const Segment* segs, uint3
     }
 }
 
+// A tile with at most TE / kSparseDensity selected elements (about 1.6 %)
+// is written by the sparse kernel (k_batch_sparse) instead of the tile engine.
+constexpr uint32_t kSparseDensity = 64;
+// grid bound of the sparse kernel
+constexpr uint64_t kSparseBlocks = 2048;
+
 template <int W, int ORDER, int VPT>
 __global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, const uint64_t* goff,
-                                                     uint64_t* counts)
+                                                     const uint32_t* tcounts, uint64_t* counts)
 {
     using C = TileCfg<W, VPT>;
     __shared__ TileShared<C> sh;
@@ -266,11 +275,11 @@ __global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32
     __shared__ uint32_t s_live;
     // runs of consecutive tiles per XCD (see xcd_runs)
     const uint64_t g = xcd_runs(blockIdx.x, gridDim.x);
-    // The real tile count, the segment search and the tile's global offset are
-    // requested together (the search reads only prefixes below nseg, and g is
-    // below the grid, the max_tiles words of goff), so the tile waits one
-    // round trip less before its mask load; a surplus tile of the host's
-    // bound leaves after the barrier.
+    // The real tile count, the segment search, the tile's global offset and
+    // its count are requested together (the search reads only prefixes below
+    // nseg, and g is below the grid, the max_tiles words of goff and
+    // tcounts), so the tile waits one round trip less before its mask load; a
+    // surplus tile of the host's bound leaves after the barrier.
     if (threadIdx.x < kWave) {
         const uint64_t total = tile_prefix[nseg];
         const uint32_t s0 = find_segment(tile_prefix, nseg, g);  // wave 0 together
@@ -280,11 +289,18 @@ __global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32
         }
     }
     const uint64_t gofs = goff[g];
+    const uint32_t tc = tcounts[g];
     __syncthreads();
     if (!s_live) return;  // uniform
     const uint32_t s = s_seg;
-    const Segment seg = segs[s];
     const uint64_t first = tile_prefix[s];
+    if (tc <= C::TE / kSparseDensity) {
+        // empty, or sparse and written by k_batch_sparse: only the segment's
+        // last tile has something to do here, the segment's count
+        if (threadIdx.x == 0 && g + 1 == tile_prefix[s + 1]) counts[s] = gofs - goff[first] + tc;
+        return;
+    }
+    const Segment seg = segs[s];
     TileArgs a{};
     a.in = static_cast<const uint8_t*>(seg.in);
     a.mask = seg.mask;
@@ -296,6 +312,97 @@ __global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32
         tile_body<W, ORDER, false, kModeGiven, true, VPT>(a, sh, g - first);
     else
         tile_body<W, ORDER, true, kModeGiven, true, VPT>(a, sh, g - first);
+}
+
+// The sparse tiles (1 <= count <= TE / kSparseDensity): a persistent grid of
+// W waves, tiles interleaved across them -- lane l of wave w reads the count
+// of tile base + l W, base = w, w + 64 W, ... -- so that the consecutive
+// sparse tiles of one column spread over many waves instead of queueing in
+// one.  Each sparse tile in the wave's ballot is written in turn: its segment
+// from the wave-cooperative search, then a lane per 16-byte mask vector (128
+// elements) of the tile, ranks from a wave scan, the selected elements
+// gathered (up to four loads in flight) and stored one by one -- no LDS, so
+// the tile costs a few wave instructions instead of a workgroup of the tile
+// engine waiting on its loads.
+template <int W, int ORDER, int VPT>
+__global__ __launch_bounds__(256) void k_batch_sparse(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, const uint32_t* tcounts,
+                                                      const uint64_t* goff)
+{
+    constexpr uint64_t TE = TileCfg<W, VPT>::TE;
+    constexpr uint32_t VEC = (uint32_t)(TE / 128);
+    constexpr uint32_t THR = (uint32_t)(TE / kSparseDensity);
+    const uint32_t lane = lane_id();
+    const uint64_t total = tile_prefix[nseg];
+    const uint64_t wave = (blockIdx.x * 256ull + threadIdx.x) / kWave;
+    const uint64_t waves = gridDim.x * (256ull / kWave);
+    for (uint64_t base = wave; base < total; base += kWave * waves) {
+        const uint64_t gl = base + lane * waves;
+        const uint32_t tc = gl < total ? tcounts[gl] : 0u;
+        uint64_t todo = __ballot(tc >= 1 && tc <= THR);
+        while (todo) {
+            const uint64_t g = base + (uint64_t)__builtin_ctzll(todo) * waves;
+            todo &= todo - 1;
+            const uint32_t s = find_segment(tile_prefix, nseg, g);
+            const uint64_t s_first = tile_prefix[s];
+            const Segment seg = segs[s];
+            const uint64_t n = seg.n, nbytes = (n + 7) / 8;
+            const uint64_t e_t = (g - s_first) * TE;
+            const bool aligned = ((uintptr_t)seg.mask & 15) == 0;
+            const uint8_t* __restrict__ in = static_cast<const uint8_t*>(seg.in);
+            uint8_t* __restrict__ out = static_cast<uint8_t*>(seg.out);
+            uint64_t r0 = goff[g] - goff[s_first];  // the tile's offset in its segment
+            for (uint32_t v0 = 0; v0 < VEC; v0 += kWave) {
+                const uint32_t v = v0 + lane;
+                const uint64_t el = e_t + 128ull * v;
+                uint32_t w[4] = {0, 0, 0, 0};
+                if (v < VEC && el < n) {
+                    const u32x4 x = aligned ? load_mask16<true>(seg.mask, el / 8, nbytes) : load_mask16<false>(seg.mask, el / 8, nbytes);
+                    w[0] = to_lsb_word<ORDER>(x.x);
+                    w[1] = to_lsb_word<ORDER>(x.y);
+                    w[2] = to_lsb_word<ORDER>(x.z);
+                    w[3] = to_lsb_word<ORDER>(x.w);
+                    if (el + 128 > n) {
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            const uint64_t b = el + 32 * k;
+                            w[k] = b >= n ? 0u : low_bits(w[k], (uint32_t)(n - b));
+                        }
+                    }
+                }
+                const uint32_t cnt = popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
+                const uint32_t inc = wave_inclusive_sum_u32(cnt);
+                uint64_t r = r0 + inc - cnt;
+                r0 += (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    uint32_t bits = w[k];
+                    const uint8_t* src = in + (el + 32 * k) * W;
+                    while (bits) {
+                        const uint32_t q0 = __builtin_ctz(bits);
+                        bits &= bits - 1;
+                        const bool h1 = bits != 0;
+                        const uint32_t q1 = h1 ? __builtin_ctz(bits) : 0u;
+                        bits &= h1 ? bits - 1 : bits;
+                        const bool h2 = bits != 0;
+                        const uint32_t q2 = h2 ? __builtin_ctz(bits) : 0u;
+                        bits &= h2 ? bits - 1 : bits;
+                        const bool h3 = bits != 0;
+                        const uint32_t q3 = h3 ? __builtin_ctz(bits) : 0u;
+                        bits &= h3 ? bits - 1 : bits;
+                        const uint64_t x0 = load_elem<W>(src + q0 * W);
+                        const uint64_t x1 = h1 ? load_elem<W>(src + q1 * W) : 0ull;
+                        const uint64_t x2 = h2 ? load_elem<W>(src + q2 * W) : 0ull;
+                        const uint64_t x3 = h3 ? load_elem<W>(src + q3 * W) : 0ull;
+                        store_elem<W>(out + r * W, x0);
+                        if (h1) store_elem<W>(out + (r + 1) * W, x1);
+                        if (h2) store_elem<W>(out + (r + 2) * W, x2);
+                        if (h3) store_elem<W>(out + (r + 3) * W, x3);
+                        r += 1u + h1 + h2 + h3;
+                    }
+                }
+            }
+        }
+    }
 }
 
 uint64_t batch_tile_elems(uint32_t w, uint32_t vpt) { return 256ull * vpt * (16ull / w); }
@@ -319,7 +426,13 @@ static hipError_t batch_impl(const Segment* segs, uint32_t nseg, uint64_t max_ti
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_scan<uint64_t><<<(unsigned)status_words, kScanBlock, 0, s>>>(tcounts, goff, max_tiles, nullptr, hdr, status, tile_prefix + nseg);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_batch_write<W, ORDER, VPT><<<(unsigned)max_tiles, 256, 0, s>>>(segs, nseg, tile_prefix, goff, counts);
+    k_batch_write<W, ORDER, VPT><<<(unsigned)max_tiles, 256, 0, s>>>(segs, nseg, tile_prefix, goff, tcounts, counts);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // the sparse tiles: a persistent grid resident at once (8 workgroups per
+    // CU; with the count kernel's larger grid the waves queue in rounds and a
+    // batch without sparse tiles took 1 % longer)
+    const uint64_t g4 = g1 < kSparseBlocks ? g1 : kSparseBlocks;
+    k_batch_sparse<W, ORDER, VPT><<<(unsigned)g4, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts, goff);
     return hipGetLastError();
 }
 

@@ -125,6 +125,46 @@ def test_batch_unaligned_segments(O, w):
                 check(cols, outs, counts, w)
 
 
+@pytest.mark.parametrize("w", [1, 2, 4, 8])
+def test_batch_sparse_tiles(O, w):
+    """Columns whose tiles hold at most 1/64 selected elements go to the
+    sparse kernel: half the columns at 0.1-1.5 % selectivity, some with tiles
+    at exactly that threshold (a periodic mask, one element in 64), among
+    dense columns, at every tile size; every column against the oracle."""
+    rng = np.random.default_rng(80 + w)
+    T2 = TILE[w] // 4
+    lengths = [int(x) for x in rng.integers(1, 12 * T2, 36)]
+    for order in (oracle.MSB_FIRST, oracle.LSB_FIRST):
+        cols, ins, masks, outs = [], [], [], []
+        for i, n in enumerate(lengths):
+            data = O.fill_values(n, NP[w], seed=300 + 7 * i + w)
+            if i % 6 == 5:
+                mask = O.mask_periodic(n, 0x80000000, order)[: (n + 7) // 8]  # 1 in 32 ...
+                mask = mask & np.tile(np.array([0xFF, 0xFF, 0xFF, 0xFF, 0, 0, 0, 0], np.uint8), mask.size // 8 + 1)[: mask.size]  # ... in every other word: 1 in 64
+            elif i % 2:
+                mask = O.mask_bernoulli(n, float(rng.uniform(0.001, 0.015)), seed=301 + 7 * i + w, bit_order=order)
+            else:
+                mask = O.mask_bernoulli(n, float(rng.uniform(0.05, 0.95)), seed=301 + 7 * i + w, bit_order=order)
+            expect, k = O.compact(data, mask, n, order)
+            cols.append((data, mask, expect, k))
+            ins.append(dev(data))
+            masks.append(dev(mask))
+            outs.append(torch.full((n + 4,), -1, dtype=ins[-1].dtype, device="cuda"))
+        table = csx.segment_table(ins, masks, outs, lengths)
+        total = sum(lengths)
+        ws = csx.alloc_workspace(csx.batch_workspace_size(len(lengths), total, w))
+        counts = torch.zeros(len(lengths), dtype=torch.int64, device="cuda")
+        for vpt in (2, 4, 8):
+            for cb in (3, 4096):
+                print(f"batch sparse tiles w={w} order={order} vpt={vpt} count_blocks={cb}", flush=True)
+                for o in outs:
+                    o.fill_(-1)
+                csx.compact_batch_config(table, w, total, csx.Config(csx.PIPELINE_THREE_PHASE, 0, vpt, cb), counts=counts, workspace=ws,
+                                         bit_order=order)
+                torch.cuda.synchronize()
+                check(cols, outs, counts, w)
+
+
 def test_batch_many_log_uniform_columns(O):
     rng = np.random.default_rng(4)
     lengths = [int(x) for x in np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 16), 300))]
