@@ -21,7 +21,9 @@ import os
 import torch
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libcsx.so")
+# CSX_LIBRARY names another build of the library (the host-sanitized one of
+# the CPU tests: python -m gpu_compressstore2_amd.build --asan)
+LIB_PATH = os.environ.get("CSX_LIBRARY") or os.path.join(PKG, "libcsx.so")
 TUNED_TABLE = os.path.join(PKG, "tuned_mi355x.txt")
 
 MSB_FIRST = 0

@@ -99,6 +99,44 @@ def _build_driver():
         shutil.move(tmp, target)
 
 
+ASAN_LIB = os.path.join(BUILD, "asan", "libcsx_asan.so")
+
+
+def build_asan(jobs=None):
+    """The library with AddressSanitizer on its host-code units (the C ABI's
+    argument checks, workspace planning, tuning-table parser and the serving
+    loop: csx_api.cpp, csx_server.cpp), -fsanitize=address after -Xarch_host,
+    linked with the regular objects of the kernel units: for the CPU tests
+    only.  Run it with the clang ASan runtime preloaded and CSX_LIBRARY
+    naming it."""
+    if not os.path.exists(HIPCC):
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    units = _units()
+    if not all(os.path.exists(os.path.join(BUILD, obj)) for _, obj, _ in units):
+        build(jobs)
+    out = os.path.dirname(ASAN_LIB)
+    os.makedirs(out, exist_ok=True)
+    san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+    objs = []
+    for src, obj, flags in units:
+        if src.endswith(".hip"):
+            objs.append(os.path.join(BUILD, obj))
+            continue
+        o = os.path.join(out, obj)
+        cmd = [HIPCC, *COMMON, "-O1", "-g", *flags, *san, "-c", os.path.join(CSRC, src), "-o", o]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        objs.append(o)
+    tmp = ASAN_LIB + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *san, "-o", tmp, *objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    shutil.move(tmp, ASAN_LIB)
+    return ASAN_LIB
+
+
 def _link(target, objs):
     tmp = target + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
@@ -109,5 +147,5 @@ def _link(target, objs):
 
 
 if __name__ == "__main__":
-    print(build())
+    print(build_asan() if "--asan" in sys.argv[1:] else build())
     sys.exit(0)

@@ -16,7 +16,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "_build", "libcsx_oracle.so")
+# CSX_ORACLE_LIBRARY names another build of the restatement (the sanitized one)
+ORACLE_SO = os.environ.get("CSX_ORACLE_LIBRARY") or os.path.join(HERE, "_build", "libcsx_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libcsx_ref.so")
 
 MSB_FIRST = 0
