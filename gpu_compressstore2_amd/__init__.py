@@ -259,20 +259,21 @@ class WriteoutArgs(ctypes.Structure):
 
     _fields_ = [
         ("input", _vp), ("mask", _vp), ("out", _vp), ("offsets", _vp), ("offsets_u32", _int), ("counts", _vp), ("out_base", _vp),
-        ("n", _u64), ("elem_size", _u32), ("chunk_len", _u32), ("bit_order", _int), ("variant", _int),
+        ("n", _u64), ("elem_size", _u32), ("chunk_len", _u32), ("bit_order", _int), ("variant", _int), ("block_size", _u32),
     ]
 
 
 def writeout(inp, mask, offsets, n, chunk_len=1024, bit_order=MSB_FIRST, variant=WRITEOUT_STAGED, counts=None, out=None, stream=None,
-             out_base=None):
+             out_base=None, block_size=0):
     """Phase 3: selected elements of inp to out at the chunk offsets (int64
     offsets, or int32 offsets as the reference keeps them); out_base is an
-    optional one-element device tensor added to every output index."""
+    optional one-element device tensor added to every output index;
+    block_size the staged tile's workgroup size (0: 256)."""
     _require_device(inp, mask, offsets, counts, out, out_base)
     if out is None:
         out = torch.empty(max(n, 1), dtype=inp.dtype, device=inp.device)
     wa = WriteoutArgs(_ptr(inp), _ptr(mask), _ptr(out), _ptr(offsets), int(offsets.element_size() == 4), _ptr(counts), _ptr(out_base),
-                      n, inp.element_size(), chunk_len, bit_order, variant)
+                      n, inp.element_size(), chunk_len, bit_order, variant, block_size)
     _check(lib().csx_writeout_ex(ctypes.byref(wa), _stream(stream)))
     return out
 
@@ -317,12 +318,14 @@ def compact(inp, mask, n=None, chunk_len=1024, bit_order=MSB_FIRST, pipeline=PIP
 
 # ---------------------------------------------------------------- tuning
 class Config(ctypes.Structure):
-    """csx_config: pipeline, chunk length, tile vectors per thread, count-kernel grid bound (batched)."""
+    """csx_config: pipeline, chunk length, tile vectors per thread, count-kernel
+    grid bound (batched), tile workgroup size (0: 256)."""
 
-    _fields_ = [("pipeline", _int), ("chunk_len", _u32), ("vpt", _u32), ("count_blocks", _u32)]
+    _fields_ = [("pipeline", _int), ("chunk_len", _u32), ("vpt", _u32), ("count_blocks", _u32), ("block_size", _u32)]
 
     def as_dict(self):
-        return {"pipeline": self.pipeline, "chunk_len": self.chunk_len, "vpt": self.vpt, "count_blocks": self.count_blocks}
+        return {"pipeline": self.pipeline, "chunk_len": self.chunk_len, "vpt": self.vpt, "count_blocks": self.count_blocks,
+                "block_size": self.block_size or 256}
 
 
 def auto_workspace_size(n, elem_size):

@@ -87,6 +87,14 @@ csx_status check_order(int order)
     return CSX_OK;
 }
 
+// a configuration's workgroup size: 128, 256 or 512, or 0 for 256
+csx_status check_block(uint32_t block)
+{
+    if (block != 0 && block != 128 && block != 256 && block != 512)
+        return fail(CSX_ERR_INVALID, "workgroup size %u (128, 256 or 512; 0 for 256)", block);
+    return CSX_OK;
+}
+
 csx_status check_n(uint64_t n)
 {
     if (n > CSX_MAX_ELEMENTS) return fail(CSX_ERR_INVALID, "element count %llu exceeds the maximum %llu", (unsigned long long)n,
@@ -128,8 +136,8 @@ ScanPlan plan_scan(uint64_t chunks)
 // u32 unit offsets[units] | u32 block totals[blocks] | u64 block prefixes[blocks].
 // The pipelines share the header and the status words.  Units (of the
 // three-phase and sparse pipelines' popcount) are min(chunk, writeout tile)
-// long; the smallest tile is 8192 / w elements, so there are at most
-// n / min(chunk, 8192 / w) of them.
+// long; the smallest tile (128 threads x 2 vectors) is 4096 / w elements, so
+// there are at most n / min(chunk, 4096 / w) of them.
 struct CompactPlan {
     uint64_t fused_tiles, scan_tiles, chunks, groups, units;
     uint32_t log2g;
@@ -151,7 +159,7 @@ CompactPlan plan_compact(uint64_t n, uint32_t w, uint32_t chunk_len)
     p.offsets_off = align_up(p.counts_off + p.chunks * 4);
     p.tcounts_off = align_up(p.offsets_off + p.chunks * 8);
     p.gsums_off = align_up(p.tcounts_off + p.fused_tiles * 4);
-    const uint64_t umin = std::min<uint64_t>(chunk_len, 8192 / w);
+    const uint64_t umin = std::min<uint64_t>(chunk_len, 4096 / w);
     p.units = (n + umin - 1) / umin;
     p.uoffs_off = align_up(p.gsums_off + p.groups * 8);
     p.btot_off = align_up(p.uoffs_off + p.units * 4);
@@ -178,15 +186,19 @@ bool overlaps(const void* a, uint64_t na, const void* b, uint64_t nb)
     return x >= y ? x - y < nb : y - x < na;
 }
 
-hipError_t dispatch_tile(uint32_t w, int mode, bool staged, int order, bool aligned, const csx::TileArgs& a, hipStream_t s, int vpt = 8)
+hipError_t dispatch_tile(uint32_t w, int mode, bool staged, int order, bool aligned, const csx::TileArgs& a, hipStream_t s, int vpt = 8,
+                         int block = 256)
 {
     switch (w) {
-        case 1: return csx::launch_tile<1>(mode, staged, order, aligned, a, s, vpt);
-        case 2: return csx::launch_tile<2>(mode, staged, order, aligned, a, s, vpt);
-        case 4: return csx::launch_tile<4>(mode, staged, order, aligned, a, s, vpt);
-        default: return csx::launch_tile<8>(mode, staged, order, aligned, a, s, vpt);
+        case 1: return csx::launch_tile<1>(mode, staged, order, aligned, a, s, vpt, block);
+        case 2: return csx::launch_tile<2>(mode, staged, order, aligned, a, s, vpt, block);
+        case 4: return csx::launch_tile<4>(mode, staged, order, aligned, a, s, vpt, block);
+        default: return csx::launch_tile<8>(mode, staged, order, aligned, a, s, vpt, block);
     }
 }
+
+// a configuration's workgroup size: 0 is the default, 256
+uint32_t block_of(uint32_t b) { return b ? b : 256; }
 
 hipError_t dispatch_tile_counts(uint32_t w, int order, bool aligned, const uint8_t* mask, uint64_t n, uint32_t* tcounts, uint64_t* gsums,
                                 uint32_t log2g, csx::WsHeader* hdr, hipStream_t s)
@@ -309,7 +321,7 @@ TuneTable& tune_table()
     return t;
 }
 
-constexpr csx_config kDefaultConfig = {CSX_PIPELINE_THREE_PHASE, 1024, 8, 0};
+constexpr csx_config kDefaultConfig = {CSX_PIPELINE_THREE_PHASE, 1024, 8, 0, 256};
 
 uint32_t log2_floor(uint64_t v)
 {
@@ -358,7 +370,7 @@ csx_config batch_config(uint32_t w)
     TuneTable& t = tune_table();
     std::lock_guard<std::mutex> g(t.mu);
     if (w < 9 && t.batch[w].vpt) return t.batch[w];
-    return csx_config{CSX_PIPELINE_THREE_PHASE, 0, 8, 4096};
+    return csx_config{CSX_PIPELINE_THREE_PHASE, 0, 8, 4096, 256};
 }
 
 // csx_predicate -> one range test; returns the element size (0 if invalid)
@@ -554,10 +566,16 @@ csx_status csx_writeout_ex(const csx_writeout_args* wa, hipStream_t stream)
         CSX_HIP(timed(stream, [&] { return dispatch_sparse(elem_size, bit_order, mask_aligned, a, stream); }), "csx_writeout launch");
         return ok();
     }
-    const uint64_t tiles = (n + tile_elems(elem_size) - 1) / tile_elems(elem_size);
+    if ((s = check_block(wa->block_size))) return s;
+    const uint32_t block = block_of(wa->block_size);
+    if (block != 256 && variant != CSX_WRITEOUT_STAGED) return fail(CSX_ERR_INVALID, "only the staged writeout takes a workgroup size other than 256");
+    // the staged tile: block threads x 8 vectors
+    const uint64_t te = (uint64_t)block * 8 * (16 / elem_size);
+    const uint64_t tiles = (n + te - 1) / te;
     if (!launch_fits(tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)tiles);
     const bool aligned = (((uintptr_t)wa->in | (uintptr_t)wa->mask) & 15) == 0;
-    CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeWriteout, variant == CSX_WRITEOUT_STAGED, bit_order, aligned, a, stream); }),
+    CSX_HIP(timed(stream,
+                  [&] { return dispatch_tile(elem_size, csx::kModeWriteout, variant == CSX_WRITEOUT_STAGED, bit_order, aligned, a, stream, 8, (int)block); }),
             "csx_writeout launch");
     return ok();
 }
@@ -618,9 +636,10 @@ static csx_status check_vpt(uint32_t vpt, uint32_t elem_size)
     return CSX_OK;
 }
 
+
 static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, uint32_t chunk_len,
-                               int bit_order, int pipeline, uint32_t vpt, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes,
-                               hipStream_t stream, bool profile = true)
+                               int bit_order, int pipeline, uint32_t vpt, uint32_t block, uint64_t* d_count, uint64_t* h_count, void* d_ws,
+                               size_t ws_bytes, hipStream_t stream, bool profile = true)
 {
     // the dominant kernel between csx_profile_begin's event pairs (creating
     // the events is the only resource this path acquires; `profile` = false
@@ -631,8 +650,11 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
     if (!known_pipeline(pipeline)) return fail(CSX_ERR_INVALID, "unknown pipeline %d", pipeline);
     if (pipeline == CSX_PIPELINE_SINGLE && n > CSX_SINGLE_MAX_ELEMENTS)
         return fail(CSX_ERR_UNSUPPORTED, "the single-launch pipeline takes at most %llu elements", (unsigned long long)CSX_SINGLE_MAX_ELEMENTS);
-    if ((s = check_vpt(vpt, elem_size))) return s;
+    if ((s = check_vpt(vpt, elem_size)) || (s = check_block(block))) return s;
     if (pipeline != CSX_PIPELINE_THREE_PHASE && vpt != 8) return fail(CSX_ERR_INVALID, "only the three-phase writeout takes a tile size other than 8");
+    block = block_of(block);
+    if (pipeline != CSX_PIPELINE_THREE_PHASE && block != 256)
+        return fail(CSX_ERR_INVALID, "only the three-phase writeout takes a workgroup size other than 256");
     const CompactPlan p = plan_compact(n, elem_size, chunk_len);
     if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
     if (n > 0 && (!d_in || !d_mask || !d_out)) return fail(CSX_ERR_INVALID, "null input, mask or output pointer");
@@ -641,7 +663,8 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
     if (((uintptr_t)d_in % elem_size) || ((uintptr_t)d_out % elem_size))
         return fail(CSX_ERR_INVALID, "input and output must be aligned to the element size");
     // the writeout's grid is one workgroup per tile of the chosen size
-    const uint64_t wtiles = (n + 256ull * vpt * (16 / elem_size) - 1) / (256ull * vpt * (16 / elem_size));
+    const uint64_t wte = (uint64_t)block * vpt * (16 / elem_size);
+    const uint64_t wtiles = (n + wte - 1) / wte;
     if (!launch_fits(p.fused_tiles) || !launch_fits(wtiles))
         return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)std::max(p.fused_tiles, wtiles));
 
@@ -718,7 +741,7 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
         auto uoffs = reinterpret_cast<uint32_t*>(base + p.uoffs_off);
         auto btotals = reinterpret_cast<uint32_t*>(base + p.btot_off);
         auto bprefix = reinterpret_cast<uint64_t*>(base + p.bpre_off);
-        const uint64_t te = pipeline == CSX_PIPELINE_SPARSE ? csx::kSparseStepElems : 256ull * vpt * (16 / elem_size);
+        const uint64_t te = pipeline == CSX_PIPELINE_SPARSE ? csx::kSparseStepElems : wte;
         const uint32_t log2unit = std::min(log2u(chunk_len), log2u(te));
         const uint32_t log2b = csx::popb_log2(n);
         CSX_HIP(csx::launch_popc_blocks(d_mask, n, log2b, log2unit, uoffs, btotals, bit_order, stream, hdr), "csx_compact popc");
@@ -737,7 +760,7 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
             CSX_HIP(timed(stream, [&] { return dispatch_sparse(elem_size, bit_order, mask_aligned, a, stream); }), "csx_compact sparse writeout");
         }
         else {
-            CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeWriteout, true, bit_order, aligned, a, stream, (int)vpt); }),
+            CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeWriteout, true, bit_order, aligned, a, stream, (int)vpt, (int)block); }),
                     "csx_compact writeout");
         }
     }
@@ -748,15 +771,15 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
 csx_status csx_compact(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, uint32_t chunk_len, int bit_order,
                        int pipeline, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream)
 {
-    return compact_impl(d_in, d_mask, d_out, n, elem_size, chunk_len, bit_order, pipeline, 8, d_count, h_count, d_ws, ws_bytes, stream);
+    return compact_impl(d_in, d_mask, d_out, n, elem_size, chunk_len, bit_order, pipeline, 8, 256, d_count, h_count, d_ws, ws_bytes, stream);
 }
 
 csx_status csx_compact_config(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, int bit_order,
                               const csx_config* cfg, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream)
 {
     if (!cfg) return fail(CSX_ERR_INVALID, "null configuration");
-    return compact_impl(d_in, d_mask, d_out, n, elem_size, cfg->chunk_len, bit_order, cfg->pipeline, cfg->vpt, d_count, h_count, d_ws, ws_bytes,
-                        stream);
+    return compact_impl(d_in, d_mask, d_out, n, elem_size, cfg->chunk_len, bit_order, cfg->pipeline, cfg->vpt, cfg->block_size, d_count, h_count,
+                        d_ws, ws_bytes, stream);
 }
 
 } // extern "C"
@@ -766,8 +789,8 @@ csx_status compact_in_workspace(const void* d_in, const uint8_t* d_mask, void* d
                                 const csx_config* cfg, uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream)
 {
     if (!cfg) return fail(CSX_ERR_INVALID, "null configuration");
-    return compact_impl(d_in, d_mask, d_out, n, elem_size, cfg->chunk_len, bit_order, cfg->pipeline, cfg->vpt, d_count, h_count, d_ws, ws_bytes,
-                        stream, false);
+    return compact_impl(d_in, d_mask, d_out, n, elem_size, cfg->chunk_len, bit_order, cfg->pipeline, cfg->vpt, cfg->block_size, d_count, h_count,
+                        d_ws, ws_bytes, stream, false);
 }
 } // namespace csx
 
@@ -1002,14 +1025,14 @@ struct BatchPlan {
     size_t prefix_off, tcounts_off, goff_off, status_off, total;
 };
 
-// the workspace is sized for the smallest tile (vpt 2), so every batched
-// configuration fits in csx_batch_workspace_size's answer
-constexpr uint32_t kBatchMinVpt = 2;
+// the workspace is sized for the smallest tile (128 threads x 2 vectors), so
+// every batched configuration fits in csx_batch_workspace_size's answer
+constexpr uint32_t kBatchMinVpt = 2, kBatchMinBlock = 128;
 
-static BatchPlan plan_batch(uint32_t nseg, uint64_t max_total, uint32_t w, uint32_t vpt)
+static BatchPlan plan_batch(uint32_t nseg, uint64_t max_total, uint32_t w, uint32_t vpt, uint32_t block)
 {
     BatchPlan p;
-    p.max_tiles = csx::batch_max_tiles(w, nseg, max_total, vpt);
+    p.max_tiles = csx::batch_max_tiles(w, nseg, max_total, vpt, block);
     p.status_words = csx::scan_tiles(p.max_tiles);
     p.prefix_off = sizeof(csx::WsHeader);
     p.tcounts_off = align_up(p.prefix_off + 8ull * (nseg + 1));
@@ -1024,7 +1047,7 @@ csx_status csx_batch_workspace_size(uint32_t nseg, uint64_t max_total_elements, 
     csx_status s;
     if ((s = check_width(elem_size)) || (s = check_n(max_total_elements))) return s;
     if (!bytes) return fail(CSX_ERR_INVALID, "null size pointer");
-    *bytes = plan_batch(nseg, max_total_elements, elem_size, kBatchMinVpt).total;
+    *bytes = plan_batch(nseg, max_total_elements, elem_size, kBatchMinVpt, kBatchMinBlock).total;
     return ok();
 }
 
@@ -1034,9 +1057,11 @@ static csx_status batch_impl(const csx_segment* d_segments, uint32_t nseg, uint6
     csx_status s;
     if ((s = check_width(elem_size)) || (s = check_order(bit_order)) || (s = check_n(max_total_elements))) return s;
     if (cfg.vpt != 2 && cfg.vpt != 4 && cfg.vpt != 8) return fail(CSX_ERR_INVALID, "batched tile of %u vectors per thread (2, 4 or 8)", cfg.vpt);
+    if ((s = check_block(cfg.block_size))) return s;
     if (nseg == 0) return ok();
     if (!d_segments || !d_counts) return fail(CSX_ERR_INVALID, "null segment table or counts pointer");
-    const BatchPlan p = plan_batch(nseg, max_total_elements, elem_size, cfg.vpt);
+    const uint32_t block = block_of(cfg.block_size);
+    const BatchPlan p = plan_batch(nseg, max_total_elements, elem_size, cfg.vpt, block);
     if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
     if (!launch_fits(p.max_tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)p.max_tiles);
     auto base = static_cast<uint8_t*>(d_ws);
@@ -1045,7 +1070,7 @@ static csx_status batch_impl(const csx_segment* d_segments, uint32_t nseg, uint6
                 return csx::launch_batch(elem_size, bit_order, reinterpret_cast<const csx::Segment*>(d_segments), nseg, p.max_tiles, d_counts,
                                          reinterpret_cast<uint64_t*>(base + p.prefix_off), reinterpret_cast<uint32_t*>(base + p.tcounts_off),
                                          reinterpret_cast<uint64_t*>(base + p.goff_off), reinterpret_cast<csx::WsHeader*>(base),
-                                         reinterpret_cast<uint64_t*>(base + p.status_off), stream, cfg.vpt, count_blocks);
+                                         reinterpret_cast<uint64_t*>(base + p.status_off), stream, cfg.vpt, count_blocks, block);
             }),
             "csx_compact_batch");
     return ok();
@@ -1302,10 +1327,14 @@ csx_status csx_compact_predicate(const void* d_in, uint64_t n, const csx_predica
         const bool aligned = (((uintptr_t)d_in | (uintptr_t)mask) & 15) == 0;
         // the writeout's tile size: the tuned three-phase choice for this width
         // and size at unknown selectivity (the chunk offsets serve any tile size)
-        uint32_t vpt = 8;
+        uint32_t vpt = 8, block = 256;
         const csx_config tc = lookup_config(w, n, -1.0);
-        if (tc.pipeline == CSX_PIPELINE_THREE_PHASE && launch_fits((n + 256ull * tc.vpt * (16 / w) - 1) / (256ull * tc.vpt * (16 / w)))) vpt = tc.vpt;
-        CSX_HIP(timed(stream, [&] { return dispatch_tile(w, csx::kModeWriteout, true, bit_order, aligned, a, stream, (int)vpt); }),
+        const uint64_t tte = (uint64_t)block_of(tc.block_size) * tc.vpt * (16 / w);
+        if (tc.pipeline == CSX_PIPELINE_THREE_PHASE && launch_fits((n + tte - 1) / tte)) {
+            vpt = tc.vpt;
+            block = block_of(tc.block_size);
+        }
+        CSX_HIP(timed(stream, [&] { return dispatch_tile(w, csx::kModeWriteout, true, bit_order, aligned, a, stream, (int)vpt, (int)block); }),
                 "csx_compact_predicate writeout");
     }
     if (h_count) return finish_host_count(h_count, count_dst, hdr, stream);
@@ -1334,7 +1363,8 @@ csx_status csx_compact_auto(const void* d_in, const uint8_t* d_mask, void* d_out
     csx_status s;
     if ((s = check_width(elem_size))) return s;
     const csx_config c = lookup_config(elem_size, n, selectivity);
-    return compact_impl(d_in, d_mask, d_out, n, elem_size, c.chunk_len, bit_order, c.pipeline, c.vpt, d_count, h_count, d_ws, ws_bytes, stream);
+    return compact_impl(d_in, d_mask, d_out, n, elem_size, c.chunk_len, bit_order, c.pipeline, c.vpt, c.block_size, d_count, h_count, d_ws, ws_bytes,
+                        stream);
 }
 
 extern "C++" {
@@ -1403,7 +1433,11 @@ csx_status tune_candidates(const void* in, const uint8_t* mask, void* out, uint6
     std::vector<csx_config> cand;
     for (uint32_t chunk : {256u, 1024u, 4096u})
         for (uint32_t vpt : {2u, 4u, 8u, 16u})
-            if (vpt != 16 || elem_size >= 2) cand.push_back(csx_config{CSX_PIPELINE_THREE_PHASE, chunk, vpt, 0});
+            if (vpt != 16 || elem_size >= 2) cand.push_back(csx_config{CSX_PIPELINE_THREE_PHASE, chunk, vpt, 0, 256});
+    // the other workgroup sizes at the usual chunk length
+    for (uint32_t block : {128u, 512u})
+        for (uint32_t vpt : {2u, 4u, 8u, 16u})
+            if (vpt != 16 || elem_size >= 2) cand.push_back(csx_config{CSX_PIPELINE_THREE_PHASE, 1024, vpt, 0, block});
     cand.push_back(csx_config{CSX_PIPELINE_TWO_PASS, 1024, 8, 0});
     cand.push_back(csx_config{CSX_PIPELINE_SPARSE, 1024, 8, 0});
     cand.push_back(csx_config{CSX_PIPELINE_SPARSE, 4096, 8, 0});
@@ -1417,7 +1451,10 @@ csx_status tune_candidates(const void* in, const uint8_t* mask, void* out, uint6
         const csx_config c = cand[i];
         double t = 0.0;
         csx_status s = time_median(
-            [&] { return compact_impl(in, mask, out, n, elem_size, c.chunk_len, bit_order, c.pipeline, c.vpt, d_count, nullptr, ws, ws_bytes, stream); },
+            [&] {
+                return compact_impl(in, mask, out, n, elem_size, c.chunk_len, bit_order, c.pipeline, c.vpt, c.block_size, d_count, nullptr, ws, ws_bytes,
+                                    stream);
+            },
             reps, stream, &t);
         if (s) return s;
         uint64_t kc = 0;
@@ -1494,19 +1531,23 @@ csx_status csx_autotune_batch(const csx_segment* d_segments, uint32_t nseg, uint
     if (!best || reps == 0) return fail(CSX_ERR_INVALID, "null result pointer or reps = 0");
     double best_t = 0.0;
     bool have = false;
-    for (uint32_t vpt : {2u, 4u, 8u})
-        for (uint32_t cb : {1024u, 2048u, 4096u, 8192u}) {
-            const csx_config c{CSX_PIPELINE_THREE_PHASE, 0, vpt, cb};
-            double t = 0.0;
-            s = time_median([&] { return batch_impl(d_segments, nseg, max_total_elements, elem_size, bit_order, c, d_counts, d_ws, ws_bytes, stream); },
-                            reps, stream, &t);
-            if (s) return s;
-            if (!have || t < best_t) {
-                best_t = t;
-                *best = c;
-                have = true;
-            }
+    std::vector<csx_config> cand;
+    for (uint32_t vpt : {2u, 4u, 8u}) {
+        for (uint32_t cb : {1024u, 2048u, 4096u, 8192u}) cand.push_back(csx_config{CSX_PIPELINE_THREE_PHASE, 0, vpt, cb, 256});
+        for (uint32_t block : {128u, 512u})
+            for (uint32_t cb : {4096u, 8192u}) cand.push_back(csx_config{CSX_PIPELINE_THREE_PHASE, 0, vpt, cb, block});
+    }
+    for (const csx_config& c : cand) {
+        double t = 0.0;
+        s = time_median([&] { return batch_impl(d_segments, nseg, max_total_elements, elem_size, bit_order, c, d_counts, d_ws, ws_bytes, stream); },
+                        reps, stream, &t);
+        if (s) return s;
+        if (!have || t < best_t) {
+            best_t = t;
+            *best = c;
+            have = true;
         }
+    }
     {
         TuneTable& t = tune_table();
         std::lock_guard<std::mutex> g(t.mu);
@@ -1523,12 +1564,13 @@ csx_status csx_tune_save(const char* path)
     if (!f) return fail(CSX_ERR_INVALID, "cannot open %s for writing", path);
     TuneTable& t = tune_table();
     std::lock_guard<std::mutex> g(t.mu);
-    fprintf(f, "# csx tuning table 1\n# compact <elem_size> <log2 n> <selectivity> <pipeline> <chunk_len> <vpt> <ms>\n"
-               "# batch <elem_size> <vpt> <count_blocks>\n");
+    fprintf(f, "# csx tuning table 2\n# compact <elem_size> <log2 n> <selectivity> <pipeline> <chunk_len> <vpt> <block_size> <ms>\n"
+               "# batch <elem_size> <vpt> <count_blocks> <block_size>\n");
     for (const TuneCell& c : t.cells)
-        fprintf(f, "compact %u %u %.6f %d %u %u %.6f\n", c.w, c.log2n, c.sel, c.cfg.pipeline, c.cfg.chunk_len, c.cfg.vpt, c.ms);
+        fprintf(f, "compact %u %u %.6f %d %u %u %u %.6f\n", c.w, c.log2n, c.sel, c.cfg.pipeline, c.cfg.chunk_len, c.cfg.vpt, block_of(c.cfg.block_size),
+                c.ms);
     for (uint32_t w : {1u, 2u, 4u, 8u})
-        if (t.batch[w].vpt) fprintf(f, "batch %u %u %u\n", w, t.batch[w].vpt, t.batch[w].count_blocks);
+        if (t.batch[w].vpt) fprintf(f, "batch %u %u %u %u\n", w, t.batch[w].vpt, t.batch[w].count_blocks, block_of(t.batch[w].block_size));
     const bool bad = ferror(f) != 0;
     if (fclose(f) != 0 || bad) return fail(CSX_ERR_INVALID, "writing %s failed", path);
     return ok();
@@ -1555,26 +1597,35 @@ csx_status csx_tune_load(const char* path)
         const char* row = line + strspn(line, " \t\r\n");  // blank lines and comments may be indented
         if (*row == '#' || *row == '\0') continue;
         TuneCell c{};
-        uint32_t w, vpt, cb;
+        uint32_t w, vpt, cb, block = 256;
         int end = -1, bend = -1;  // offsets past the row's fields and trailing blanks: the row must end there
-        if (sscanf(row, "compact %u %u %lf %d %u %u %lf %n", &c.w, &c.log2n, &c.sel, &c.cfg.pipeline, &c.cfg.chunk_len, &c.cfg.vpt, &c.ms,
-                   &end) == 7 &&
-            end >= 0 && row[end] == '\0') {
+        // rows without a workgroup size (version 1 of the format) mean 256; the
+        // shorter form is tried first, and it must end the row, so that a
+        // version-2 row's integer block size is never read as the time
+        const bool compact7 = sscanf(row, "compact %u %u %lf %d %u %u %lf %n", &c.w, &c.log2n, &c.sel, &c.cfg.pipeline, &c.cfg.chunk_len, &c.cfg.vpt,
+                                     &c.ms, &end) == 7 &&
+                              end >= 0 && row[end] == '\0';
+        const bool compact8 = !compact7 && (end = -1, sscanf(row, "compact %u %u %lf %d %u %u %u %lf %n", &c.w, &c.log2n, &c.sel, &c.cfg.pipeline,
+                                                             &c.cfg.chunk_len, &c.cfg.vpt, &block, &c.ms, &end) == 8) &&
+                              end >= 0 && row[end] == '\0';
+        if (compact7 || compact8) {
+            c.cfg.block_size = block;
             if (check_width(c.w) || check_chunk(c.cfg.chunk_len) || check_vpt(c.cfg.vpt, c.w) || c.log2n > 40 || !(c.sel >= 0 && c.sel <= 1) ||
-                !(c.ms >= 0 && std::isfinite(c.ms)) ||
+                !(c.ms >= 0 && std::isfinite(c.ms)) || (block != 128 && block != 256 && block != 512) ||
                 !known_pipeline(c.cfg.pipeline) || (c.cfg.pipeline == CSX_PIPELINE_SINGLE && c.log2n > 24) ||
-                (c.cfg.pipeline != CSX_PIPELINE_THREE_PHASE && c.cfg.vpt != 8)) {
+                (c.cfg.pipeline != CSX_PIPELINE_THREE_PHASE && (c.cfg.vpt != 8 || block != 256))) {
                 st = fail(CSX_ERR_INVALID, "%s:%u: configuration out of range", path, lineno);
                 break;
             }
             cells.push_back(c);
         }
-        else if (sscanf(row, "batch %u %u %u %n", &w, &vpt, &cb, &bend) == 3 && bend >= 0 && row[bend] == '\0') {
-            if (check_width(w) || (vpt != 2 && vpt != 4 && vpt != 8) || cb == 0) {
+        else if ((sscanf(row, "batch %u %u %u %n", &w, &vpt, &cb, &bend) == 3 && bend >= 0 && row[bend] == '\0') ||
+                 (bend = -1, sscanf(row, "batch %u %u %u %u %n", &w, &vpt, &cb, &block, &bend) == 4 && bend >= 0 && row[bend] == '\0')) {
+            if (check_width(w) || (vpt != 2 && vpt != 4 && vpt != 8) || cb == 0 || (block != 128 && block != 256 && block != 512)) {
                 st = fail(CSX_ERR_INVALID, "%s:%u: batch configuration out of range", path, lineno);
                 break;
             }
-            batch.push_back({w, csx_config{CSX_PIPELINE_THREE_PHASE, 0, vpt, cb}});
+            batch.push_back({w, csx_config{CSX_PIPELINE_THREE_PHASE, 0, vpt, cb, block}});
         }
         else {
             st = fail(CSX_ERR_INVALID, "%s:%u: unreadable line", path, lineno);
@@ -1595,7 +1646,7 @@ csx_status csx_tune_clear(void)
     TuneTable& t = tune_table();
     std::lock_guard<std::mutex> g(t.mu);
     t.cells.clear();
-    for (csx_config& c : t.batch) c = csx_config{0, 0, 0, 0};
+    for (csx_config& c : t.batch) c = csx_config{0, 0, 0, 0, 0};
     return ok();
 }
 

@@ -180,10 +180,10 @@ __device__ __forceinline__ void batch_counts_packed(const Segment* segs, uint32_
     }
 }
 
-template <int W, int ORDER, int VPT>
+template <int W, int ORDER, int VPT, int BLOCK>
 __global__ __launch_bounds__(256) void k_batch_counts(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, uint32_t* tcounts)
 {
-    constexpr uint64_t TE = TileCfg<W, VPT>::TE;
+    constexpr uint64_t TE = TileCfg<W, VPT, BLOCK>::TE;
     constexpr uint32_t VEC_PER_TILE = (uint32_t)(TE / 128);
     if constexpr (VEC_PER_TILE < kWave) {
         batch_counts_packed<W, ORDER, VEC_PER_TILE, TE>(segs, nseg, tile_prefix, tcounts);
@@ -265,11 +265,11 @@ constexpr uint32_t kSparseDensity = 64;
 // grid bound of the sparse kernel
 constexpr uint64_t kSparseBlocks = 2048;
 
-template <int W, int ORDER, int VPT>
-__global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, const uint64_t* goff,
-                                                     const uint32_t* tcounts, uint64_t* counts)
+template <int W, int ORDER, int VPT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batch_write(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, const uint64_t* goff,
+                                                       const uint32_t* tcounts, uint64_t* counts)
 {
-    using C = TileCfg<W, VPT>;
+    using C = TileCfg<W, VPT, BLOCK>;
     __shared__ TileShared<C> sh;
     __shared__ uint32_t s_seg;
     __shared__ uint32_t s_live;
@@ -309,9 +309,9 @@ __global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32
     a.given_off = gofs - goff[first];
     a.count_out = counts + s;
     if ((((uintptr_t)a.in) | ((uintptr_t)a.mask)) & 15)
-        tile_body<W, ORDER, false, kModeGiven, true, VPT>(a, sh, g - first);
+        tile_body<W, ORDER, false, kModeGiven, true, VPT, BLOCK>(a, sh, g - first);
     else
-        tile_body<W, ORDER, true, kModeGiven, true, VPT>(a, sh, g - first);
+        tile_body<W, ORDER, true, kModeGiven, true, VPT, BLOCK>(a, sh, g - first);
 }
 
 // The sparse tiles (1 <= count <= TE / kSparseDensity): a persistent grid of
@@ -324,11 +324,11 @@ __global__ __launch_bounds__(256) void k_batch_write(const Segment* segs, uint32
 // gathered (up to four loads in flight) and stored one by one -- no LDS, so
 // the tile costs a few wave instructions instead of a workgroup of the tile
 // engine waiting on its loads.
-template <int W, int ORDER, int VPT>
+template <int W, int ORDER, int VPT, int BLOCK>
 __global__ __launch_bounds__(256) void k_batch_sparse(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, const uint32_t* tcounts,
                                                       const uint64_t* goff)
 {
-    constexpr uint64_t TE = TileCfg<W, VPT>::TE;
+    constexpr uint64_t TE = TileCfg<W, VPT, BLOCK>::TE;
     constexpr uint32_t VEC = (uint32_t)(TE / 128);
     constexpr uint32_t THR = (uint32_t)(TE / kSparseDensity);
     const uint32_t lane = lane_id();
@@ -405,67 +405,82 @@ __global__ __launch_bounds__(256) void k_batch_sparse(const Segment* segs, uint3
     }
 }
 
-uint64_t batch_tile_elems(uint32_t w, uint32_t vpt) { return 256ull * vpt * (16ull / w); }
+uint64_t batch_tile_elems(uint32_t w, uint32_t vpt, uint32_t block) { return (uint64_t)block * vpt * (16ull / w); }
 
-uint64_t batch_max_tiles(uint32_t w, uint32_t nseg, uint64_t max_total, uint32_t vpt)
+uint64_t batch_max_tiles(uint32_t w, uint32_t nseg, uint64_t max_total, uint32_t vpt, uint32_t block)
 {
-    return max_total / batch_tile_elems(w, vpt) + nseg;
+    return max_total / batch_tile_elems(w, vpt, block) + nseg;
 }
 
-template <int W, int ORDER, int VPT>
+template <int W, int ORDER, int VPT, int BLOCK>
 static hipError_t batch_impl(const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix, uint32_t* tcounts,
                              uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks, hipStream_t s)
 {
     const uint64_t status_words = (max_tiles + kScanTile - 1) / kScanTile;
-    k_batch_plan<<<1, 256, 0, s>>>(segs, nseg, TileCfg<W, VPT>::TE, max_tiles, tile_prefix, counts, hdr, status, status_words);
+    k_batch_plan<<<1, 256, 0, s>>>(segs, nseg, TileCfg<W, VPT, BLOCK>::TE, max_tiles, tile_prefix, counts, hdr, status, status_words);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || max_tiles == 0) return e;
     uint64_t g1 = (max_tiles + 3) / 4;
     if (g1 > count_blocks) g1 = count_blocks;
-    k_batch_counts<W, ORDER, VPT><<<(unsigned)g1, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts);
+    k_batch_counts<W, ORDER, VPT, BLOCK><<<(unsigned)g1, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_scan<uint64_t><<<(unsigned)status_words, kScanBlock, 0, s>>>(tcounts, goff, max_tiles, nullptr, hdr, status, tile_prefix + nseg);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_batch_write<W, ORDER, VPT><<<(unsigned)max_tiles, 256, 0, s>>>(segs, nseg, tile_prefix, goff, tcounts, counts);
+    k_batch_write<W, ORDER, VPT, BLOCK><<<(unsigned)max_tiles, BLOCK, 0, s>>>(segs, nseg, tile_prefix, goff, tcounts, counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // the sparse tiles: a persistent grid resident at once (8 workgroups per
     // CU; with the count kernel's larger grid the waves queue in rounds and a
     // batch without sparse tiles took 1 % longer)
     const uint64_t g4 = g1 < kSparseBlocks ? g1 : kSparseBlocks;
-    k_batch_sparse<W, ORDER, VPT><<<(unsigned)g4, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts, goff);
+    k_batch_sparse<W, ORDER, VPT, BLOCK><<<(unsigned)g4, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts, goff);
     return hipGetLastError();
 }
 
-template <int W, int VPT>
+template <int W, int VPT, int BLOCK>
 static hipError_t batch_order(int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
                               uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks, hipStream_t s)
 {
-    return order == kBitMsb ? batch_impl<W, kBitMsb, VPT>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s)
-                            : batch_impl<W, kBitLsb, VPT>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+    return order == kBitMsb
+               ? batch_impl<W, kBitMsb, VPT, BLOCK>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s)
+               : batch_impl<W, kBitLsb, VPT, BLOCK>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+}
+
+template <int W, int VPT>
+static hipError_t batch_block(uint32_t block, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts,
+                              uint64_t* tile_prefix, uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks,
+                              hipStream_t s)
+{
+    switch (block) {
+        case 128: return batch_order<W, VPT, 128>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 256: return batch_order<W, VPT, 256>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 512: return batch_order<W, VPT, 512>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 template <int W>
-static hipError_t batch_vpt(uint32_t vpt, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts,
+static hipError_t batch_vpt(uint32_t vpt, uint32_t block, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts,
                             uint64_t* tile_prefix, uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks,
                             hipStream_t s)
 {
     switch (vpt) {
-        case 2: return batch_order<W, 2>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
-        case 4: return batch_order<W, 4>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
-        case 8: return batch_order<W, 8>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 2: return batch_block<W, 2>(block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 4: return batch_block<W, 4>(block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 8: return batch_block<W, 8>(block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_batch(uint32_t w, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
-                        uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s, uint32_t vpt, uint32_t count_blocks)
+                        uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s, uint32_t vpt, uint32_t count_blocks,
+                        uint32_t block)
 {
     if (count_blocks < 1) count_blocks = 1;
     switch (w) {
-        case 1: return batch_vpt<1>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
-        case 2: return batch_vpt<2>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
-        case 4: return batch_vpt<4>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
-        default: return batch_vpt<8>(vpt, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 1: return batch_vpt<1>(vpt, block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 2: return batch_vpt<2>(vpt, block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 4: return batch_vpt<4>(vpt, block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        default: return batch_vpt<8>(vpt, block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
     }
 }
 

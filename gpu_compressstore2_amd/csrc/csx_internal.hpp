@@ -69,13 +69,14 @@ hipError_t launch_stream_probe(int kind, const void* src, void* dst, uint64_t by
 // *dst = base[i-1] + total[i-1] (0 for i = 0): the multi-stream carry
 hipError_t launch_carry(uint64_t* dst, const uint64_t* base, const uint64_t* total, uint32_t i, hipStream_t s);
 // batched compaction (csx_batch.hip)
-// vpt in {2, 4, 8}: tile of 256 * vpt * 16/w elements; count_blocks bounds the
+// vpt in {2, 4, 8}, block in {128, 256, 512}: tile of block * vpt * 16/w
+// elements written by a workgroup of `block` threads; count_blocks bounds the
 // persistent grid of the per-tile count kernel
-uint64_t batch_tile_elems(uint32_t w, uint32_t vpt);
-uint64_t batch_max_tiles(uint32_t w, uint32_t nseg, uint64_t max_total, uint32_t vpt);
+uint64_t batch_tile_elems(uint32_t w, uint32_t vpt, uint32_t block);
+uint64_t batch_max_tiles(uint32_t w, uint32_t nseg, uint64_t max_total, uint32_t vpt, uint32_t block);
 hipError_t launch_batch(uint32_t w, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
                         uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s, uint32_t vpt = 8,
-                        uint32_t count_blocks = 4096);
+                        uint32_t count_blocks = 4096, uint32_t block = 256);
 // device generators (csx_gen.hip)
 hipError_t launch_gen_bernoulli(uint8_t* mask, uint64_t n, uint64_t thr, uint64_t seed, int order, hipStream_t s);
 hipError_t launch_gen_pattern(uint8_t* mask, uint64_t n, uint32_t pattern, uint32_t L, int order, hipStream_t s);
@@ -93,8 +94,10 @@ hipError_t launch_validate(const void* a, const void* b, uint64_t len, uint32_t 
 
 // tile engine, instantiated once per element width (csx_tile.hip)
 // vpt: 16-byte vectors per thread of the three-phase writeout tile (2, 4, 8, or 16
-// for W >= 2); the other modes use kTileVpt
-template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool aligned, const TileArgs& a, hipStream_t s, int vpt = kTileVpt<W>);
+// for W >= 2), block: its workgroup size (128, 256, 512); the other modes use
+// kTileVpt and 256
+template <int W>
+hipError_t launch_tile(int mode, bool staged, int order, bool aligned, const TileArgs& a, hipStream_t s, int vpt = kTileVpt<W>, int block = 256);
 template <int W>
 hipError_t launch_tile_counts(int order, bool aligned, const uint8_t* mask, uint64_t n, uint32_t* tcounts, uint64_t* gsums, uint32_t log2g,
                               WsHeader* hdr, hipStream_t s);
@@ -102,10 +105,10 @@ template <int W> hipError_t launch_thread_per_chunk(int order, const TileArgs& a
 // sparse writeout (three-phase offsets, no LDS); mask_aligned: 16-byte mask pointer
 template <int W> hipError_t launch_sparse(int order, bool mask_aligned, const TileArgs& a, hipStream_t s);
 
-extern template hipError_t launch_tile<1>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
-extern template hipError_t launch_tile<2>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
-extern template hipError_t launch_tile<4>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
-extern template hipError_t launch_tile<8>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
+extern template hipError_t launch_tile<1>(int, bool, int, bool, const TileArgs&, hipStream_t, int, int);
+extern template hipError_t launch_tile<2>(int, bool, int, bool, const TileArgs&, hipStream_t, int, int);
+extern template hipError_t launch_tile<4>(int, bool, int, bool, const TileArgs&, hipStream_t, int, int);
+extern template hipError_t launch_tile<8>(int, bool, int, bool, const TileArgs&, hipStream_t, int, int);
 extern template hipError_t launch_tile_counts<1>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, WsHeader*, hipStream_t);
 extern template hipError_t launch_thread_per_chunk<1>(int, const TileArgs&, hipStream_t);
 extern template hipError_t launch_sparse<1>(int, bool, const TileArgs&, hipStream_t);

@@ -342,8 +342,8 @@ __global__ __launch_bounds__(kPopbThreads) void k_popc_blocks(const uint8_t* __r
 //     16-byte stores (element stores only in the two boundary vectors that
 //     the neighbouring tiles share).  !STAGED: each selected element is
 //     stored straight from registers to out[offset + rank].
-template <int W, int VPT_ = kTileVpt<W>> struct TileCfg {
-    static constexpr int BLOCK = 256;
+template <int W, int VPT_ = kTileVpt<W>, int BLOCK_ = 256> struct TileCfg {
+    static constexpr int BLOCK = BLOCK_;             // threads of the tile's workgroup (128, 256 or 512)
     static constexpr int VPT = VPT_;                 // 16-byte data vectors per thread
     static constexpr int EPV = 16 / W;               // elements per vector
     static constexpr int TE = BLOCK * VPT * EPV;     // elements per tile
@@ -457,10 +457,10 @@ __device__ __forceinline__ uint64_t prefix_finish(const TileArgs& a, uint64_t tb
 // One tile: every thread of the block calls this with the same `tile`.
 // Contains __syncthreads; LDS in `sh` may be reused once all threads return
 // and pass a barrier.
-template <int W, int ORDER, bool ALIGNED, int MODE, bool STAGED, int VPT>
-__device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<W, VPT>>& sh, uint64_t tile)
+template <int W, int ORDER, bool ALIGNED, int MODE, bool STAGED, int VPT, int BLOCK = 256>
+__device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<W, VPT, BLOCK>>& sh, uint64_t tile)
 {
-    using C = TileCfg<W, VPT>;
+    using C = TileCfg<W, VPT, BLOCK>;
     const uint32_t t = threadIdx.x;
     const uint64_t n = a.n;
     const uint64_t nbytes = (n + 7) / 8;
@@ -776,10 +776,10 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
 }
 
 // One tile per workgroup: tile = blockIdx.x (writeout) or a ticket (fused).
-template <int W, int ORDER, bool ALIGNED, int MODE, bool STAGED, int VPT = kTileVpt<W>>
-__global__ __launch_bounds__(256) void k_tile(TileArgs a)
+template <int W, int ORDER, bool ALIGNED, int MODE, bool STAGED, int VPT = kTileVpt<W>, int BLOCK = 256>
+__global__ __launch_bounds__(BLOCK) void k_tile(TileArgs a)
 {
-    using C = TileCfg<W, VPT>;
+    using C = TileCfg<W, VPT, BLOCK>;
     __shared__ TileShared<C> sh;
     uint64_t tile;
     if constexpr (MODE == kModeFused) {
@@ -797,7 +797,7 @@ __global__ __launch_bounds__(256) void k_tile(TileArgs a)
     else {
         tile = blockIdx.x;
     }
-    tile_body<W, ORDER, ALIGNED, MODE, STAGED, VPT>(a, sh, tile);
+    tile_body<W, ORDER, ALIGNED, MODE, STAGED, VPT, BLOCK>(a, sh, tile);
 }
 
 // ======================================================================

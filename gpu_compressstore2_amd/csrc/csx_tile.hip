@@ -9,10 +9,21 @@
 
 namespace csx {
 
-template <int W, int ORDER, bool ALIGNED, int VPT> static void launch_writeout_vpt(const TileArgs& a, hipStream_t s)
+template <int W, int ORDER, bool ALIGNED, int VPT, int BLOCK> static void launch_writeout_shape(const TileArgs& a, hipStream_t s)
 {
-    const uint64_t tiles = (a.n + TileCfg<W, VPT>::TE - 1) / TileCfg<W, VPT>::TE;
-    launch_k(k_tile<W, ORDER, ALIGNED, kModeWriteout, true, VPT>, dim3((unsigned)tiles), dim3(256), s, a);
+    using C = TileCfg<W, VPT, BLOCK>;
+    const uint64_t tiles = (a.n + C::TE - 1) / C::TE;
+    launch_k(k_tile<W, ORDER, ALIGNED, kModeWriteout, true, VPT, BLOCK>, dim3((unsigned)tiles), dim3(BLOCK), s, a);
+}
+
+// the staged writeout's tile: `block` threads (128, 256 or 512) x VPT vectors
+template <int W, int ORDER, bool ALIGNED, int VPT> static void launch_writeout_vpt(const TileArgs& a, int block, hipStream_t s)
+{
+    switch (block) {
+        case 128: launch_writeout_shape<W, ORDER, ALIGNED, VPT, 128>(a, s); break;
+        case 512: launch_writeout_shape<W, ORDER, ALIGNED, VPT, 512>(a, s); break;
+        default: launch_writeout_shape<W, ORDER, ALIGNED, VPT, 256>(a, s); break;
+    }
 }
 
 template <int W, bool ALIGNED, int VPT> static void launch_pattern_vpt(const TileArgs& a, hipStream_t s)
@@ -22,7 +33,7 @@ template <int W, bool ALIGNED, int VPT> static void launch_pattern_vpt(const Til
 }
 
 template <int W, int ORDER, bool ALIGNED>
-static void launch_tile_impl(int mode, bool staged, int vpt, const TileArgs& a, unsigned grid, hipStream_t s)
+static void launch_tile_impl(int mode, bool staged, int vpt, int block, const TileArgs& a, unsigned grid, hipStream_t s)
 {
     if (mode == kModeTwoPass) {
         launch_k(k_tile<W, ORDER, ALIGNED, kModeTwoPass, true>, dim3(grid), dim3(256), s, a);
@@ -54,12 +65,12 @@ static void launch_tile_impl(int mode, bool staged, int vpt, const TileArgs& a, 
         // three-phase writeout: a tile's offset comes from the chunk offsets,
         // so any tile size works with the same workspace
         switch (vpt) {
-            case 2: launch_writeout_vpt<W, ORDER, ALIGNED, 2>(a, s); break;
-            case 4: launch_writeout_vpt<W, ORDER, ALIGNED, 4>(a, s); break;
+            case 2: launch_writeout_vpt<W, ORDER, ALIGNED, 2>(a, block, s); break;
+            case 4: launch_writeout_vpt<W, ORDER, ALIGNED, 4>(a, block, s); break;
             case 16:
-                if constexpr (W >= 2) launch_writeout_vpt<W, ORDER, ALIGNED, 16>(a, s);
+                if constexpr (W >= 2) launch_writeout_vpt<W, ORDER, ALIGNED, 16>(a, block, s);
                 break;
-            default: launch_k(k_tile<W, ORDER, ALIGNED, kModeWriteout, true>, dim3(grid), dim3(256), s, a); break;
+            default: launch_writeout_vpt<W, ORDER, ALIGNED, 8>(a, block, s); break;
         }
     }
     else {
@@ -67,23 +78,26 @@ static void launch_tile_impl(int mode, bool staged, int vpt, const TileArgs& a, 
     }
 }
 
-template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool aligned, const TileArgs& a, hipStream_t s, int vpt)
+template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool aligned, const TileArgs& a, hipStream_t s, int vpt, int block)
 {
     if (vpt != 2 && vpt != 4 && vpt != 8 && !(vpt == 16 && W >= 2)) return hipErrorInvalidValue;
+    // only the staged writeout's tile takes another workgroup size than 256
+    if (block != 128 && block != 256 && block != 512) return hipErrorInvalidValue;
+    if (block != 256 && !(mode == kModeWriteout && staged)) return hipErrorInvalidValue;
     const uint64_t tiles = (a.n + kTileElems<W> - 1) / kTileElems<W>;
     if (tiles == 0) return hipSuccess;
     const unsigned grid = (unsigned)tiles;
     if (order == kBitMsb) {
         if (aligned)
-            launch_tile_impl<W, kBitMsb, true>(mode, staged, vpt, a, grid, s);
+            launch_tile_impl<W, kBitMsb, true>(mode, staged, vpt, block, a, grid, s);
         else
-            launch_tile_impl<W, kBitMsb, false>(mode, staged, vpt, a, grid, s);
+            launch_tile_impl<W, kBitMsb, false>(mode, staged, vpt, block, a, grid, s);
     }
     else {
         if (aligned)
-            launch_tile_impl<W, kBitLsb, true>(mode, staged, vpt, a, grid, s);
+            launch_tile_impl<W, kBitLsb, true>(mode, staged, vpt, block, a, grid, s);
         else
-            launch_tile_impl<W, kBitLsb, false>(mode, staged, vpt, a, grid, s);
+            launch_tile_impl<W, kBitLsb, false>(mode, staged, vpt, block, a, grid, s);
     }
     return hipGetLastError();
 }
@@ -144,7 +158,7 @@ template <int W> hipError_t launch_sparse(int order, bool mask_aligned, const Ti
     return hipGetLastError();
 }
 
-template hipError_t launch_tile<CSX_TILE_W>(int, bool, int, bool, const TileArgs&, hipStream_t, int);
+template hipError_t launch_tile<CSX_TILE_W>(int, bool, int, bool, const TileArgs&, hipStream_t, int, int);
 template hipError_t launch_sparse<CSX_TILE_W>(int, bool, const TileArgs&, hipStream_t);
 template hipError_t launch_tile_counts<CSX_TILE_W>(int, bool, const uint8_t*, uint64_t, uint32_t*, uint64_t*, uint32_t, WsHeader*, hipStream_t);
 template hipError_t launch_thread_per_chunk<CSX_TILE_W>(int, const TileArgs&, hipStream_t);

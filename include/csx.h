@@ -155,6 +155,8 @@ typedef struct {
     uint32_t chunk_len;
     int bit_order;
     int variant;
+    uint32_t block_size; /* workgroup size of the STAGED variant's tile: 128, 256 or 512 (0: 256); the
+                            reference's block size (switch_3pass_proc_true_striding, kernel_3pass.cuh:491-533) */
 } csx_writeout_args;
 csx_status csx_writeout_ex(const csx_writeout_args* args, hipStream_t stream);
 csx_status csx_writeout_w1(const uint8_t* d_in, const uint8_t* d_mask, const uint64_t* d_offsets, const uint32_t* d_counts, uint8_t* d_out,
@@ -286,8 +288,11 @@ uint32_t csx_server_inflight(const csx_server* server);
  * A compaction configuration.  pipeline: CSX_PIPELINE_*; chunk_len: the
  * three-phase chunk length; vpt: tile size in 16-byte vectors per thread of
  * the three-phase writeout (2, 4, 8, or 16 when elem_size >= 2; TWO_PASS,
- * FUSED and SPARSE take 8, which they do not use) or of the batched form (2, 4 or 8); count_blocks: grid
- * bound of the batched form's per-tile count kernel (0: 4096).
+ * FUSED, SPARSE and SINGLE take 8, which they do not use) or of the batched
+ * form (2, 4 or 8); count_blocks: grid bound of the batched form's per-tile
+ * count kernel (0: 4096); block_size: the workgroup size of those tiles
+ * (128, 256 or 512; 0 means 256; the other pipelines take 0 or 256) -- the
+ * reference's block-size argument (src/kernels/kernel_3pass.cuh:491-533).
  * The tuning table maps (elem_size, log2 n, selectivity) to the fastest
  * measured configuration; csx_tune_lookup answers with the measured cell of
  * that width nearest in log2 n, then in selectivity, or with the default
@@ -299,6 +304,8 @@ typedef struct {
     uint32_t chunk_len;
     uint32_t vpt;
     uint32_t count_blocks;
+    uint32_t block_size; /* threads per workgroup of the staged writeout tile (THREE_PHASE) or of the batched
+                            write kernel: 128, 256 or 512 (0: 256); the tile is block_size * vpt 16-byte vectors */
 } csx_config;
 /* csx_compact with an explicit configuration; workspace csx_workspace_size(n, elem_size, cfg->chunk_len) */
 csx_status csx_compact_config(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, int bit_order,

@@ -70,11 +70,11 @@ def test_batch_every_tile_size_and_count_grid(O, w):
         ws = csx.alloc_workspace(csx.batch_workspace_size(len(lengths), total, w))
         counts = torch.zeros(len(lengths), dtype=torch.int64, device="cuda")
         for vpt in (2, 4, 8):
-            for cb in (1, 7, 64, 4096):
-                print(f"batch w={w} order={order} vpt={vpt} count_blocks={cb}", flush=True)
+            for cb, block in ((1, 256), (7, 128), (64, 512), (4096, 256), (4096, 128), (4096, 512)):
+                print(f"batch w={w} order={order} vpt={vpt} count_blocks={cb} block={block}", flush=True)
                 for o in outs:
                     o.fill_(-1)
-                csx.compact_batch_config(table, w, total, csx.Config(csx.PIPELINE_THREE_PHASE, 0, vpt, cb), counts=counts, workspace=ws,
+                csx.compact_batch_config(table, w, total, csx.Config(csx.PIPELINE_THREE_PHASE, 0, vpt, cb, block), counts=counts, workspace=ws,
                                          bit_order=order)
                 torch.cuda.synchronize()
                 check(cols, outs, counts, w)
@@ -155,11 +155,11 @@ def test_batch_sparse_tiles(O, w):
         ws = csx.alloc_workspace(csx.batch_workspace_size(len(lengths), total, w))
         counts = torch.zeros(len(lengths), dtype=torch.int64, device="cuda")
         for vpt in (2, 4, 8):
-            for cb in (3, 4096):
-                print(f"batch sparse tiles w={w} order={order} vpt={vpt} count_blocks={cb}", flush=True)
+            for cb, block in ((3, 128), (4096, 256), (4096, 512)):
+                print(f"batch sparse tiles w={w} order={order} vpt={vpt} count_blocks={cb} block={block}", flush=True)
                 for o in outs:
                     o.fill_(-1)
-                csx.compact_batch_config(table, w, total, csx.Config(csx.PIPELINE_THREE_PHASE, 0, vpt, cb), counts=counts, workspace=ws,
+                csx.compact_batch_config(table, w, total, csx.Config(csx.PIPELINE_THREE_PHASE, 0, vpt, cb, block), counts=counts, workspace=ws,
                                          bit_order=order)
                 torch.cuda.synchronize()
                 check(cols, outs, counts, w)

@@ -187,11 +187,14 @@ def test_writeout_variants(O, w, variant):
             expect, ke = O.compact(data, mask, n, order)
             assert total == ke
             canary = np.frombuffer(np.full((n + 16) * w, 0x5A, np.uint8).tobytes(), dtype=NP[w]).copy()
-            d_out = dev(canary)
-            csx.writeout(dev(data), dev(mask), dev(offs), n, chunk, order, variant, counts=dev(counts) if skip else None, out=d_out)
-            got = host(d_out, NP[w])
-            assert (got[:ke] == expect).all()
-            assert (got[ke:] == canary[ke:]).all()
+            # the staged tile at every workgroup size, the other variants at 256
+            for block in ((0, 128, 512) if variant == csx.WRITEOUT_STAGED else (0,)):
+                d_out = dev(canary)
+                csx.writeout(dev(data), dev(mask), dev(offs), n, chunk, order, variant, counts=dev(counts) if skip else None, out=d_out,
+                             block_size=block)
+                got = host(d_out, NP[w])
+                assert (got[:ke] == expect).all(), block
+                assert (got[ke:] == canary[ke:]).all(), block
 
 
 def test_unaligned_pointers(O):
@@ -272,6 +275,29 @@ def test_two_pass_many_groups(O):
     expect, ke = O.compact(data, mask, n)
     got, k = run_compact(data, mask, n, oracle.MSB_FIRST, csx.PIPELINE_TWO_PASS, canary=False)
     assert k == ke and (got == expect).all()
+
+
+@pytest.mark.parametrize("block", [128, 512])
+@pytest.mark.parametrize("w", [1, 2, 4, 8])
+def test_three_phase_workgroup_sizes(O, w, block):
+    """The three-phase pipeline with 128- and 512-thread tiles at every tile
+    size, both bit orders, ragged sizes and chunk lengths below and above the
+    tile (units of min(chunk, tile))."""
+    for vpt in ((2, 4, 8) if w == 1 else (2, 4, 8, 16)):
+        te = block * vpt * (16 // w)
+        for n, chunk, kind in ((te * 3 + 17, 1024, "bern50"), (te - 1, 64, "bern99"), (5 * te + 1, 1 << 17, "runs"), (777, 256, "bern01")):
+            for order in (oracle.MSB_FIRST, oracle.LSB_FIRST):
+                print(f"three-phase block={block} vpt={vpt} w={w} n={n} chunk={chunk} mask={kind} order={order}", flush=True)
+                data = O.fill_values(n, NP[w], seed=n + vpt)
+                mask = dirty_tail(make_mask(O, kind, n, n + block, order), n, order)
+                expect, ke = O.compact(data, mask, n, order)
+                canary = np.frombuffer(np.full((n + 16) * w, 0xC3, np.uint8).tobytes(), dtype=NP[w]).copy()
+                d_out = dev(canary)
+                _, k = csx.compact_config(dev(data), dev(mask), csx.Config(csx.PIPELINE_THREE_PHASE, chunk, vpt, 0, block), n=n, bit_order=order,
+                                          out=d_out)
+                got = host(d_out, NP[w])
+                assert k == ke and (got[:k] == expect).all()
+                assert (got[k:] == canary[k:]).all()
 
 
 def test_single_launch_at_its_cap():

@@ -19,10 +19,11 @@ VIEW = {1: np.uint8, 2: np.int16, 4: np.int32, 8: np.int64}
 def random_config(rng, w):
     pipeline = int(rng.choice([csx.PIPELINE_FUSED, csx.PIPELINE_THREE_PHASE, csx.PIPELINE_TWO_PASS, csx.PIPELINE_SPARSE, csx.PIPELINE_SINGLE]))
     chunk = int(rng.choice([32, 64, 128, 256, 1024, 4096, 8192, 1 << 15]))
-    vpt = 8
+    vpt, block = 8, 0
     if pipeline == csx.PIPELINE_THREE_PHASE:
         vpt = int(rng.choice([2, 4, 8, 16] if w >= 2 else [2, 4, 8]))
-    return csx.Config(pipeline, chunk, vpt, 0)
+        block = int(rng.choice([0, 128, 256, 512]))
+    return csx.Config(pipeline, chunk, vpt, 0, block)
 
 
 @pytest.mark.parametrize("block", range(8))

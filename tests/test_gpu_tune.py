@@ -17,7 +17,11 @@ def candidates(w):
     for chunk in (256, 1024, 4096):
         for vpt in (2, 4, 8, 16):
             if vpt != 16 or w >= 2:
-                out.append(csx.Config(csx.PIPELINE_THREE_PHASE, chunk, vpt, 0))
+                out.append(csx.Config(csx.PIPELINE_THREE_PHASE, chunk, vpt, 0, 256))
+    for block in (128, 512):
+        for vpt in (2, 4, 8, 16):
+            if vpt != 16 or w >= 2:
+                out.append(csx.Config(csx.PIPELINE_THREE_PHASE, 1024, vpt, 0, block))
     out.append(csx.Config(csx.PIPELINE_TWO_PASS, 1024, 8, 0))
     out.append(csx.Config(csx.PIPELINE_FUSED, 1024, 8, 0))
     out.append(csx.Config(csx.PIPELINE_SPARSE, 1024, 8, 0))

@@ -72,12 +72,50 @@ def test_bad_lines_rejected(tmp_path, line):
     assert csx.tune_lookup(4, 1 << 20, 0.5).vpt == 8  # nothing half-loaded
 
 
+def test_block_size_rows_and_version_one_rows(tmp_path):
+    """Rows with a workgroup size (format 2) and without (format 1, meaning
+    256) load side by side; a save writes the sizes back."""
+    p = tmp_path / "t.txt"
+    p.write_text("compact 4 20 0.500000 1 1024 2 512 0.02\ncompact 4 24 0.500000 1 256 4 0.03\n"
+                 "batch 4 4 8192 128\nbatch 8 8 4096\n")
+    csx.tune_load(str(p))
+    c = csx.tune_lookup(4, 1 << 20, 0.5)
+    assert (c.pipeline, c.chunk_len, c.vpt, c.block_size) == (1, 1024, 2, 512)
+    c = csx.tune_lookup(4, 1 << 24, 0.5)
+    assert (c.vpt, c.block_size) == (4, 256)
+    out = tmp_path / "saved.txt"
+    csx.tune_save(str(out))
+    text = out.read_text()
+    assert "compact 4 20 0.500000 1 1024 2 512 " in text and "compact 4 24 0.500000 1 256 4 256 " in text
+    assert "batch 4 4 8192 128" in text and "batch 8 8 4096 256" in text
+
+
+@pytest.mark.parametrize("line", [
+    "compact 4 20 0.5 1 1024 2 384 0.02",  # no such workgroup size
+    "compact 4 20 0.5 2 1024 8 512 0.02",  # only the three-phase writeout takes another size
+    "batch 4 4 8192 64",
+])
+def test_bad_block_sizes_rejected(tmp_path, line):
+    p = tmp_path / "bad.txt"
+    p.write_text(line + "\n")
+    with pytest.raises(csx.CsxError):
+        csx.tune_load(str(p))
+
+
 def test_config_checks_before_device_work():
     L = csx.lib()
     for pipeline, chunk, vpt, w in [(1, 1024, 3, 4), (1, 1024, 16, 1), (2, 1024, 4, 4), (7, 1024, 8, 4), (1, 100, 8, 4)]:
         c = csx.Config(pipeline, chunk, vpt, 0)
         assert L.csx_compact_config(None, None, None, 10, w, 0, ctypes.byref(c), None, None, None, 0, None) == 1
     assert L.csx_compact_config(None, None, None, 10, 4, 0, None, None, None, None, 0, None) == 1
+    for bad_block in (64, 384, 1024):
+        c = csx.Config(1, 1024, 2, 0, bad_block)
+        assert L.csx_compact_config(None, None, None, 10, 4, 0, ctypes.byref(c), None, None, None, 0, None) == 1
+        assert b"workgroup size" in L.csx_last_error()
+    c = csx.Config(csx.PIPELINE_TWO_PASS, 1024, 8, 0, 512)  # only the three-phase writeout takes another size
+    assert L.csx_compact_config(None, None, None, 10, 4, 0, ctypes.byref(c), None, None, ctypes.c_void_p(4096), 1 << 20, None) == 1
+    c = csx.Config(1, 0, 4, 0, 96)
+    assert L.csx_compact_batch_config(None, 1, 10, 4, 0, ctypes.byref(c), None, None, 0, None) == 1
     c = csx.Config(1, 0, 16, 0)
     assert L.csx_compact_batch_config(None, 1, 10, 4, 0, ctypes.byref(c), None, None, 0, None) == 1
     ms = ctypes.c_double()
