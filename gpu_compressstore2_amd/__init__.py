@@ -110,6 +110,7 @@ SIGNATURES = {
     "csx_stream_probe": (_int, [_int, _vp, _vp, _u64, _u32, _u32, _vp]),
     "csx_profile_begin": (_int, []),
     "csx_profile_end": (_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
+    "csx_last_launch": (_int, [ctypes.POINTER(_u32), ctypes.POINTER(_u64)]),
     "csx_compact_config": (_int, [_vp, _vp, _vp, _u64, _u32, _int, _vp, _vp, _vp, _vp, _sz, _vp]),
     "csx_compact_noalloc": (_int, [_vp, _vp, _vp, _u64, _u32, _int, _vp, _vp, ctypes.POINTER(_u64), _vp]),
     "csx_auto_workspace_size": (_int, [_u64, _u32, ctypes.POINTER(_sz)]),
@@ -686,6 +687,14 @@ def profile_end():
     cnt = _u64(0)
     _check(lib().csx_profile_end(ctypes.byref(ms), ctypes.byref(cnt)))
     return ms.value, cnt.value
+
+
+def last_launch():
+    """(workgroup size, grid) of the dominant kernel of this thread's last
+    compact/writeout/pattern call; (0, 0) when it launched nothing."""
+    b, g = _u32(0), _u64(0)
+    _check(lib().csx_last_launch(ctypes.byref(b), ctypes.byref(g)))
+    return b.value, g.value
 
 
 def workspace_error(workspace, stream=None):

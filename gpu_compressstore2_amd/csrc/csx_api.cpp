@@ -249,16 +249,28 @@ constexpr size_t kMaxProfiled = 1u << 16;
 } // namespace
 namespace csx {
 thread_local LaunchEvents g_launch_events;
+thread_local LaunchShape g_launch_shape;
 }
 namespace {
 
-// When profiling: `fn` (returning hipError_t) launches exactly one kernel
-// through launch_k, whose start and end the armed event pair records.  A call
-// that launches nothing (an empty input) leaves no pair.
+// `fn` launches the call's dominant kernel through launch_k, which records
+// its grid and workgroup size for csx_last_launch (0 and 0 when it launches
+// nothing).
+template <typename F> hipError_t captured(F&& fn)
+{
+    csx::g_launch_shape = csx::LaunchShape{true, 0, 0};
+    const hipError_t e = fn();
+    csx::g_launch_shape.armed = false;
+    return e;
+}
+
+// As captured(); when profiling, the launch's start and end are also
+// recorded by the armed event pair.  A call that launches nothing (an empty
+// input) leaves no pair.
 template <typename F> hipError_t timed(hipStream_t s, F&& fn)
 {
     (void)s;
-    if (!g_prof.on || g_prof.events.size() >= kMaxProfiled) return fn();
+    if (!g_prof.on || g_prof.events.size() >= kMaxProfiled) return captured(fn);
     hipEvent_t a, b;
     hipError_t e = hipEventCreate(&a);
     if (e != hipSuccess) return e;
@@ -268,7 +280,7 @@ template <typename F> hipError_t timed(hipStream_t s, F&& fn)
         return e;
     }
     csx::g_launch_events = csx::LaunchEvents{a, b, false};
-    e = fn();
+    e = captured(fn);
     const bool used = csx::g_launch_events.used;
     csx::g_launch_events = csx::LaunchEvents{};
     if (used) {
@@ -644,7 +656,7 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
     // the dominant kernel between csx_profile_begin's event pairs (creating
     // the events is the only resource this path acquires; `profile` = false
     // keeps it out)
-    auto timed = [&](hipStream_t st, auto&& fn) -> hipError_t { return profile ? ::timed(st, fn) : fn(); };
+    auto timed = [&](hipStream_t st, auto&& fn) -> hipError_t { return profile ? ::timed(st, fn) : captured(fn); };
     csx_status s;
     if ((s = check_width(elem_size)) || (s = check_chunk(chunk_len)) || (s = check_order(bit_order)) || (s = check_n(n))) return s;
     if (!known_pipeline(pipeline)) return fail(CSX_ERR_INVALID, "unknown pipeline %d", pipeline);
@@ -1254,6 +1266,14 @@ csx_status csx_profile_end(double* total_ms, uint64_t* launches)
     if (total_ms) *total_ms = sum;
     if (launches) *launches = count;
     if (err != hipSuccess) return hip_fail(err, "csx_profile_end");
+    return ok();
+}
+
+csx_status csx_last_launch(uint32_t* block_size, uint64_t* grid)
+{
+    if (!block_size || !grid) return fail(CSX_ERR_INVALID, "null argument");
+    *block_size = csx::g_launch_shape.block;
+    *grid = csx::g_launch_shape.grid;
     return ok();
 }
 

@@ -19,8 +19,22 @@ struct LaunchEvents {
     bool used = false;
 };
 extern thread_local LaunchEvents g_launch_events;
+// the launch shape of the next kernel launched through launch_k while armed
+// (csx_last_launch reports it)
+struct LaunchShape {
+    bool armed = false;
+    uint64_t grid = 0;
+    uint32_t block = 0;
+};
+extern thread_local LaunchShape g_launch_shape;
 template <typename K, typename... A> inline void launch_k(K kernel, dim3 grid, dim3 block, hipStream_t s, A... args)
 {
+    LaunchShape& sh = g_launch_shape;
+    if (sh.armed) {
+        sh.armed = false;
+        sh.grid = (uint64_t)grid.x * grid.y * grid.z;
+        sh.block = block.x * block.y * block.z;
+    }
     LaunchEvents& ev = g_launch_events;
     if (ev.start && !ev.used) {
         ev.used = true;

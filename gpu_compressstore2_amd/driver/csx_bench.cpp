@@ -2,7 +2,11 @@
 // reference's src/main.cu: the same flags (getopt ":zuer8d:l:i:f:p:s:t:g:m:c:k:b:n:",
 // src/main.cu:197), the same inputs (seed-42 PRNG data and masks), the same
 // sweep and the same CSV columns (src/main.cu:166) followed by
-// ;bytes;gbps;pct_peak.  Every run is validated against a CPU compaction of
+// ;bytes;gbps;pct_peak;launch_block;launch_grid -- the workgroup size and grid
+// of the dominant kernel that actually ran (csx_last_launch): -b/-n set the
+// tile workgroup size of the LDS-staged writeout (bench10 and csx_three_phase;
+// nearest of 128, 256, 512), -g/-m are accepted and change nothing, the tile
+// kernels launching one workgroup per tile.  Every run is validated against a CPU compaction of
 // the same inputs, as the reference does (src/main.cu:173-182).
 //
 // Extra flags: -w <1|2|4|8> element width (default 4; -8 keeps its meaning),
@@ -273,10 +277,10 @@ template <typename T> void run(Opts o)
         return bench10_3pass_optimized_read_skipping_optimized_writeout_cub_pss(&id, d_input, d_mask, d_output, n, cs, bs, gs);
     });
     benchs.emplace_back("csx_three_phase",
-                        [&](int cs, int, int) { return bench_csx_pipeline(&id, CSX_PIPELINE_THREE_PHASE, d_input, d_mask, d_output, n, cs); });
-    benchs.emplace_back("csx_two_pass", [&](int cs, int, int) { return bench_csx_pipeline(&id, CSX_PIPELINE_TWO_PASS, d_input, d_mask, d_output, n, cs); });
-    benchs.emplace_back("csx_fused", [&](int cs, int, int) { return bench_csx_pipeline(&id, CSX_PIPELINE_FUSED, d_input, d_mask, d_output, n, cs); });
-    benchs.emplace_back("csx_sparse", [&](int cs, int, int) { return bench_csx_pipeline(&id, CSX_PIPELINE_SPARSE, d_input, d_mask, d_output, n, cs); });
+                        [&](int cs, int bs, int) { return bench_csx_pipeline(&id, CSX_PIPELINE_THREE_PHASE, d_input, d_mask, d_output, n, cs, bs); });
+    benchs.emplace_back("csx_two_pass", [&](int cs, int bs, int) { return bench_csx_pipeline(&id, CSX_PIPELINE_TWO_PASS, d_input, d_mask, d_output, n, cs, bs); });
+    benchs.emplace_back("csx_fused", [&](int cs, int bs, int) { return bench_csx_pipeline(&id, CSX_PIPELINE_FUSED, d_input, d_mask, d_output, n, cs, bs); });
+    benchs.emplace_back("csx_sparse", [&](int cs, int bs, int) { return bench_csx_pipeline(&id, CSX_PIPELINE_SPARSE, d_input, d_mask, d_output, n, cs, bs); });
     if (!o.only.empty()) {
         std::vector<std::pair<std::string, Fn>> keep;
         std::stringstream ss(o.only);
@@ -288,15 +292,23 @@ template <typename T> void run(Opts o)
     }
 
     const double bytes = (double)n * w + (double)mask_bytes + (double)out_length * w;
-    std::cout << "benchmark;chunk_length;block_size;grid_size;time_popc;time_pss1;time_pss2;time_proc;time_total;bytes;gbps;pct_peak" << std::endl;
+    std::cout << "benchmark;chunk_length;block_size;grid_size;time_popc;time_pss1;time_pss2;time_proc;time_total;bytes;gbps;pct_peak;launch_block;"
+                 "launch_grid"
+              << std::endl;
     int failures_total = 0;
     for (int grid_size = o.grid_size_min; grid_size <= o.grid_size_max; grid_size = grid_size ? grid_size * 2 : o.grid_size_max + 1) {
         for (int block_size = o.block_size_min; block_size <= o.block_size_max; block_size *= 2) {
             for (int chunk_length = o.chunk_length_min; chunk_length <= o.chunk_length_max; chunk_length *= 2) {
                 std::vector<timings> ts(benchs.size());
+                std::vector<std::pair<uint32_t, uint64_t>> shapes(benchs.size());
                 for (int it = 0; it < o.iterations; it++) {
                     for (size_t i = 0; i < benchs.size(); i++) {
                         ts[i] += benchs[i].second(chunk_length, block_size, grid_size);
+                        // rocPRIM's select is no kernel of the engine: 0;0
+                        if (benchs[i].first == "bench8_rocprim_flagged")
+                            shapes[i] = {0, 0};
+                        else
+                            check(csx_last_launch(&shapes[i].first, &shapes[i].second), "csx_last_launch");
                         uint64_t fc = 0;
                         uint64_t count = 0;
                         hip_check(hipMemcpy(&count, id.d_out_count, sizeof(count), hipMemcpyDeviceToHost), "hipMemcpy");
@@ -313,7 +325,7 @@ template <typename T> void run(Opts o)
                     timings t = ts[i] / (float)o.iterations;
                     const double gbps = t.total > 0 ? bytes / (t.total * 1e-3) / 1e9 : 0.0;
                     std::cout << benchs[i].first << ";" << chunk_length << ";" << block_size << ";" << grid_size << ";" << t << ";" << (uint64_t)bytes
-                              << ";" << gbps << ";" << 100.0 * gbps / kHbmPeakGBs << std::endl;
+                              << ";" << gbps << ";" << 100.0 * gbps / kHbmPeakGBs << ";" << shapes[i].first << ";" << shapes[i].second << std::endl;
                 }
             }
         }

@@ -21,7 +21,7 @@ import gpu_compressstore2_amd as csx
 WIDTHS = (1, 2, 4, 8)
 LOG2N = (16, 20, 22, 24, 26, 28)
 SELS = (0.001, 0.01, 0.03, 0.1, 0.25, 0.5, 0.75, 0.9, 0.99)
-PIPELINES = {0: "fused", 1: "three_phase", 2: "two_pass", 3: "sparse"}
+PIPELINES = {0: "fused", 1: "three_phase", 2: "two_pass", 3: "sparse", 4: "single"}
 
 
 def main():

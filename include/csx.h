@@ -318,8 +318,9 @@ csx_status csx_compact_auto(const void* d_in, const uint8_t* d_mask, void* d_out
                             uint64_t* d_count, uint64_t* h_count, void* d_ws, size_t ws_bytes, hipStream_t stream);
 /* Autotuner for one cell: generates n elements and a Bernoulli(selectivity)
  * mask on the device, times every candidate (THREE_PHASE x chunk {256, 1024,
- * 4096} x vpt {2, 4, 8, 16}, TWO_PASS, FUSED, SPARSE, and SINGLE up to its
- * cap) with HIP events -- `reps` calls back to back between two events, three
+ * 4096} x vpt {2, 4, 8, 16} in 256-thread tiles, THREE_PHASE at chunk 1024 x
+ * vpt in 128- and 512-thread tiles, TWO_PASS, FUSED, SPARSE at chunks 1024
+ * and 4096, and SINGLE up to its cap) with HIP events -- `reps` calls back to back between two events, three
  * rounds after one warm-up, the median round's per-call time -- checks that
  * all give the same count,
  * records the fastest in the table and returns it.  Allocates and frees its
@@ -515,6 +516,13 @@ csx_status csx_stream_probe(int kind, void* d_dst, const void* d_src, uint64_t b
  * phase with CUDA_TIME event pairs, src/cuda_time.cuh:11-21.) */
 csx_status csx_profile_begin(void);
 csx_status csx_profile_end(double* total_ms, uint64_t* launches);
+/* The workgroup size and grid (workgroups) of the dominant kernel -- the one
+ * csx_profile times -- of the calling thread's last call that reached it:
+ * what the call actually launched, for drivers that print the launch shape
+ * beside the block and grid arguments they were given (the reference's CSV
+ * columns block_size;grid_size, src/main.cu:166).  0 and 0 when that call
+ * launched nothing (empty input). */
+csx_status csx_last_launch(uint32_t* block_size, uint64_t* grid);
 
 #ifdef __cplusplus
 }

@@ -7,8 +7,12 @@
 // (print and exit, src/cuda_try.cuh:8-20), same timing convention (every
 // launcher synchronises, times its phase with an event pair on the legacy
 // stream and returns milliseconds, src/cuda_time.cuh:11-21).  Differences:
-//   * block/grid counts are accepted and ignored (launch shapes are fixed
-//     per element width for gfx950);
+//   * the block count of the LDS-staged writeout (optimized_writeout) and of
+//     the engine's three-phase pipeline becomes the tile's workgroup size,
+//     rounded to the nearest size built for gfx950 (128, 256, 512); the other
+//     kernels run 256-thread workgroups.  Grid counts are accepted and
+//     ignored: the tile kernels launch one workgroup per tile.
+//     csx_last_launch() tells what a call launched;
 //   * the selected count written to d_out_count is exact;
 //   * prepare_buffers does not modify the mask (the kernels ignore bits past n);
 //   * launchers whose reference signature has no workspace argument use a
@@ -72,6 +76,16 @@ inline void* scratch(size_t bytes, int slot = 0)
 }
 
 template <class T> constexpr uint32_t width() { return (uint32_t)sizeof(T); }
+
+// a reference block size (threads per block, src/main.cu:38-46) as the
+// engine's tile workgroup size: 0 (unset) is 256, anything else the nearest
+// built size at or above it, clamped to [128, 512]
+inline uint32_t engine_block(uint32_t threadcount)
+{
+    if (threadcount == 0) return 256;
+    if (threadcount <= 128) return 128;
+    return threadcount <= 256 ? 256 : 512;
+}
 
 // ------------------------------------------------------------ phase launchers
 
@@ -154,13 +168,14 @@ inline const uint32_t* full_offsets(uint32_t* d_pss, bool full_pss, uint32_t chu
 
 // src/kernels/kernel_3pass.cuh:536-576 (optimized_writeout: LDS-staged stores)
 template <typename T, bool optimized_writeout>
-float launch_3pass_proc_true(hipEvent_t ce_start, hipEvent_t ce_stop, uint32_t /*blockcount*/, uint32_t /*threadcount*/, T* d_input, T* d_output,
+float launch_3pass_proc_true(hipEvent_t ce_start, hipEvent_t ce_stop, uint32_t /*blockcount*/, uint32_t threadcount, T* d_input, T* d_output,
                              uint8_t* d_mask, uint32_t* d_pss, bool full_pss, uint32_t* d_popc, uint32_t chunk_length, uint32_t element_count)
 {
     const uint32_t chunks = (uint32_t)csx_chunk_count(element_count, chunk_length);
     return time_phase(ce_start, ce_stop, [&] {
-        const csx_writeout_args a = writeout_args(d_input, d_mask, d_output, full_offsets(d_pss, full_pss, chunks), d_popc, element_count,
-                                                  width<T>(), chunk_length, optimized_writeout ? CSX_WRITEOUT_STAGED : CSX_WRITEOUT_DIRECT);
+        csx_writeout_args a = writeout_args(d_input, d_mask, d_output, full_offsets(d_pss, full_pss, chunks), d_popc, element_count, width<T>(),
+                                            chunk_length, optimized_writeout ? CSX_WRITEOUT_STAGED : CSX_WRITEOUT_DIRECT);
+        a.block_size = optimized_writeout ? engine_block(threadcount) : 256;
         check(csx_writeout_ex(&a, 0), "launch_3pass_proc_true");
     });
 }
@@ -464,15 +479,22 @@ timings bench10_3pass_optimized_read_skipping_optimized_writeout_cub_pss(interme
     return t;
 }
 
-// the engine's own pipelines behind one call (no reference counterpart)
+// the engine's own pipelines behind one call (no reference counterpart); the
+// three-phase pipeline's writeout tile runs engine_block(block_size) threads
 template <class T>
-timings bench_csx_pipeline(intermediate_data* id, int pipeline, T* d_input, uint8_t* d_mask, T* d_output, size_t element_count, size_t chunk_length)
+timings bench_csx_pipeline(intermediate_data* id, int pipeline, T* d_input, uint8_t* d_mask, T* d_output, size_t element_count, size_t chunk_length,
+                           int block_size = 0)
 {
     id->prepare_buffers(element_count, chunk_length, d_output, d_mask);
+    csx_config cfg{};
+    cfg.pipeline = pipeline;
+    cfg.chunk_len = (uint32_t)chunk_length;
+    cfg.vpt = 8;
+    cfg.block_size = pipeline == CSX_PIPELINE_THREE_PHASE ? engine_block((uint32_t)block_size) : 256;
     timings t{};
     t.total = time_phase(id->start, id->stop, [&] {
-        check(csx_compact(d_input, d_mask, d_output, element_count, sizeof(T), chunk_length, CSX_MSB_FIRST, pipeline, id->d_out_count, nullptr,
-                          id->d_workspace, id->workspace_size, 0),
+        check(csx_compact_config(d_input, d_mask, d_output, element_count, sizeof(T), CSX_MSB_FIRST, &cfg, id->d_out_count, nullptr, id->d_workspace,
+                                 id->workspace_size, 0),
               "csx_compact");
     });
     return t;

@@ -276,3 +276,15 @@ def test_server_and_arena_argument_errors():
     n = ctypes.c_uint32()
     assert L.csx_server_poll(None, None, 0, ctypes.byref(n)) == 1
     assert L.csx_server_queued(None) == 0 and L.csx_server_inflight(None) == 0
+
+
+def test_last_launch_reports_nothing_on_a_fresh_thread():
+    """csx_last_launch reads the calling thread's record: (0, 0) before any
+    launch on that thread; null arguments refused (no device work)."""
+    import threading
+    got = []
+    t = threading.Thread(target=lambda: got.append(csx.last_launch()))
+    t.start()
+    t.join()
+    assert got == [(0, 0)]
+    assert csx.lib().csx_last_launch(None, None) == 1

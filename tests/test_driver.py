@@ -6,7 +6,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "gpu_compressstore2_amd", "csx_bench")
-HEADER = "benchmark;chunk_length;block_size;grid_size;time_popc;time_pss1;time_pss2;time_proc;time_total;bytes;gbps;pct_peak"
+HEADER = "benchmark;chunk_length;block_size;grid_size;time_popc;time_pss1;time_pss2;time_proc;time_total;bytes;gbps;pct_peak;launch_block;launch_grid"
 
 
 def run(args, timeout=240):
@@ -73,9 +73,39 @@ def test_driver_runs_and_validates(args):
     lines = [ln for ln in r.stdout.splitlines() if ln.strip()]
     assert lines[0] == HEADER
     rows = [ln.split(";") for ln in lines[1:]]
-    assert rows and all(len(x) == 12 for x in rows)
+    assert rows and all(len(x) == 14 for x in rows)
     names = {x[0] for x in rows}
     for must in ("bench1_base_variant", "bench8_rocprim_flagged", "bench10_3pass_optimized_read_skipping_optimized_writeout_cub_pss",
                  "csx_three_phase"):
         assert must in names
     assert all(float(x[8]) > 0 for x in rows)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w", [1, 4, 8])
+def test_driver_block_sweep_sets_the_tile_workgroup(w):
+    """-n 64 -b 1024 sweeps block sizes 64..1024: the LDS-staged writeout
+    (bench10) and the engine's three-phase pipeline run tiles of the nearest
+    built workgroup size (128, 256, 512) with one workgroup per tile of
+    block x 8 16-byte vectors; the other kernels keep 256; rocPRIM's select
+    reports 0;0; every row validated."""
+    n = 300007
+    args = ["-w", str(w), "-u", "-s", "0.5", "-l", str(n), "-i", "1", "-k", "1024", "-c", "1024", "-n", "64", "-b", "1024",
+            "-x", "bench1_base_variant,bench8_rocprim_flagged,bench10_3pass_optimized_read_skipping_optimized_writeout_cub_pss,csx_three_phase,"
+                  "csx_two_pass"]
+    print("driver", " ".join(args), flush=True)
+    r = run(args)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert "validation failure" not in r.stderr
+    rows = [ln.split(";") for ln in r.stdout.splitlines()[1:] if ln.strip()]
+    assert len(rows) == 5 * 5  # block sizes 64, 128, 256, 512, 1024
+    for x in rows:
+        name, block, launch_block, launch_grid = x[0], int(x[2]), int(x[12]), int(x[13])
+        if name == "bench8_rocprim_flagged":
+            assert (launch_block, launch_grid) == (0, 0)
+        elif name.startswith("bench10") or name == "csx_three_phase":
+            want = min(max(block, 128), 512)
+            te = want * 8 * (16 // w)
+            assert (launch_block, launch_grid) == (want, (n + te - 1) // te), x
+        else:
+            assert launch_block == 256 and launch_grid > 0, x

@@ -295,6 +295,9 @@ def test_three_phase_workgroup_sizes(O, w, block):
                 d_out = dev(canary)
                 _, k = csx.compact_config(dev(data), dev(mask), csx.Config(csx.PIPELINE_THREE_PHASE, chunk, vpt, 0, block), n=n, bit_order=order,
                                           out=d_out)
+                # the writeout ran block-thread tiles; an input of one default tile runs whole in one 256-thread workgroup
+                one_tile = 256 * 8 * (16 // w)
+                assert csx.last_launch() == ((256, 1) if n <= one_tile else (block, (n + te - 1) // te))
                 got = host(d_out, NP[w])
                 assert k == ke and (got[:k] == expect).all()
                 assert (got[k:] == canary[k:]).all()
