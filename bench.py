@@ -42,8 +42,9 @@ import gpu_compressstore2_amd as csx  # noqa: E402
 from gpu_compressstore2_amd import baseline  # noqa: E402
 
 METRIC = "effective GB/s (data+mask+out) & % HBM peak, 2^28 u32 at 1/50/99% sel"
-# counter summaries (HBM bytes per launch) of the 50 % writeout, by tile vectors per thread
-TRAFFIC_FILES = {8: "traffic_writeout_2p28_u32_p50.json", 2: "traffic_writeout_vpt2_2p28_u32_p50.json"}
+# counter summary (HBM bytes per launch of the headline's writeout kernel, calibrated
+# against csx_copy) of two `rocprofv3 --pmc` runs of `python bench.py --only headline`
+TRAFFIC_FILE = "profiles/traffic_headline_2p28_u32_p50.json"
 # kernel-trace summary of `python bench.py --only headline` (the writeout's average duration)
 PROFILE_FILE = "profiles/headline_2p28_u32_p50.md"
 HBM_PEAK_GBS = 8000.0
@@ -418,18 +419,27 @@ def main():
         if args.serve_seconds > 0:
             sweep["serving_poisson_20k"] = bench_serving(args.serve_seconds)
 
-        # plain device copy of the input size (csx_copy): what this HBM sustains
-        copy_ms = time_steps(lambda i: csx.copy(out, datas[i % ROTATE]), K, W)
-        sweep["copy_4N"] = {"GB/s": round(K * 2 * 4 * n / (copy_ms * 1e-3) / 1e9, 1), "ms_per_step": round(copy_ms / K, 5)}
+    # plain device copy of the input size (csx_copy): what this HBM sustains,
+    # and the known-size kernel the counter runs calibrate against
+    copy_steps = K if args.only != "headline" else 5
+    copy_ms = time_steps(lambda i: csx.copy(out, datas[i % ROTATE]), copy_steps, min(W, 2))
+    sweep["copy_4N"] = {"GB/s": round(copy_steps * 2 * 4 * n / (copy_ms * 1e-3) / 1e9, 1), "ms_per_step": round(copy_ms / copy_steps, 5)}
 
     gbs, ms_step, kern_avg, step_bytes, cfg = main_result
     achieved = step_bytes / (kern_avg * 1e-3) / 1e9
-    # HBM bytes per launch of this tile size's writeout, from the committed counter summaries
+    if cfg["pipeline"] == csx.PIPELINE_THREE_PHASE:
+        kernel = f"csx::k_tile<4, 0, true, 0, true, {cfg['vpt']}, {cfg['block_size']}>"
+    else:
+        kernel = f"the dominant kernel of pipeline {cfg['pipeline']}"
+    # HBM bytes per launch of that kernel, from the committed counter summary
+    # (null when it was taken for another tile shape)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", TRAFFIC_FILES.get(cfg["vpt"], ""))
+    tpath = os.path.join(ROOT, TRAFFIC_FILE)
     if os.path.isfile(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            t = json.load(f)
+        if t.get("kernel") == kernel:
+            traffic = t.get("hbm_bytes_per_launch")
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
@@ -437,7 +447,7 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "kernel": f"csx::k_tile<4, 0, true, 0, true, {cfg['vpt']}> (phase 3, staged writeout)",
+        "kernel": kernel + (" (phase 3, staged writeout)" if cfg["pipeline"] == csx.PIPELINE_THREE_PHASE else ""),
         "kernel_ms": round(kern_avg, 5),
         "algorithmic_bytes_per_launch": int(step_bytes),
         "profile": PROFILE_FILE,
@@ -462,12 +472,14 @@ def main():
         "data": "synthetic: float32 uniform [0,1) moved as u32 words, seeded torch generators, MSB-first Bernoulli masks, 3 rotating input sets",
         "config": {
             "workload": "single-column compaction of 2^28 4-byte elements, Bernoulli(0.5) mask, tuned configuration (csx_compact_auto: "
-                        f"block popcount with unit offsets, block prefixes, staged writeout; chunk {cfg['chunk_len']}, {cfg['vpt']} vectors per thread)",
+                        f"block popcount with unit offsets, block prefixes, staged writeout; chunk {cfg['chunk_len']}, {cfg['vpt']} vectors per thread, "
+                        f"{cfg['block_size']}-thread tiles)",
             "n": n,
             "elem_bytes": w,
             "selectivity": 0.5,
             "chunk_len": cfg["chunk_len"],
             "tile_vectors_per_thread": cfg["vpt"],
+            "tile_workgroup_size": cfg["block_size"],
             "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2),
         },
         "roofline": roofline,
@@ -761,17 +773,19 @@ def sweep_shapes(args):
     """Tile size against mask shape: 2^28 elements of every width; Bernoulli
     masks at 10-90 %, alternating runs of geometric mean 16-4096 (50 %) and
     Zipf blocks; the three-phase call (chunk 1024) with 2/4/8/16-vector tiles
+    of 256 threads, 2-vector tiles of 128 and 4-vector tiles of 512 threads,
     and the two-pass call, each row with its mask's occupied 64-byte-line
     ratio against a Bernoulli mask of the same density."""
     n = 1 << (args.log2n if args.log2n else 28)
     masks = {f"bernoulli_{p}": csx.gen_mask_bernoulli(n, p, seed=11) for p in (0.1, 0.25, 0.5, 0.75, 0.9)}
     masks.update({f"runs_{r}": csx.gen_mask_runs(n, r, seed=7) for r in (16, 64, 256, 1024, 4096)})
     masks["zipf_blocks"] = csx.gen_mask_zipf_blocks(n, 4096, 1.2, 65, seed=5)
-    forms = [("vpt2", csx.PIPELINE_THREE_PHASE, 2), ("vpt4", csx.PIPELINE_THREE_PHASE, 4), ("vpt8", csx.PIPELINE_THREE_PHASE, 8),
-             ("vpt16", csx.PIPELINE_THREE_PHASE, 16), ("two_pass", csx.PIPELINE_TWO_PASS, 8)]
+    forms = [("vpt2", csx.PIPELINE_THREE_PHASE, 2, 256), ("vpt4", csx.PIPELINE_THREE_PHASE, 4, 256), ("vpt8", csx.PIPELINE_THREE_PHASE, 8, 256),
+             ("vpt16", csx.PIPELINE_THREE_PHASE, 16, 256), ("vpt2_b128", csx.PIPELINE_THREE_PHASE, 2, 128),
+             ("vpt4_b512", csx.PIPELINE_THREE_PHASE, 4, 512), ("two_pass", csx.PIPELINE_TWO_PASS, 8, 256)]
     count = torch.empty(1, dtype=torch.int64, device="cuda")
     rows = []
-    print("width;mask;selected;line_ratio;" + ";".join(f"{f}_us" for f, _, _ in forms) + ";best", flush=True)
+    print("width;mask;selected;line_ratio;" + ";".join(f"{f[0]}_us" for f in forms) + ";best", flush=True)
     for w in (1, 2, 4, 8):
         data = csx.gen_values(n, w, seed=3)
         out = torch.empty_like(data)
@@ -783,17 +797,17 @@ def sweep_shapes(args):
             lines = int(((mask[: n // 8].view(-1, 1) >> sh) & 1).view(-1, epl).any(dim=1).sum().item())
             expected = (n // epl) * (1.0 - (1.0 - k / n) ** epl)
             row = {"width": w, "mask": mname, "selected": k, "line_ratio": round(lines / expected if expected else 1.0, 3)}
-            for fname, pipeline, vpt in forms:
+            for fname, pipeline, vpt, block in forms:
                 if vpt == 16 and w == 1:
                     row[f"{fname}_us"] = None
                     continue
-                cfg = csx.Config(pipeline, 1024, vpt, 0)
+                cfg = csx.Config(pipeline, 1024, vpt, 0, block)
                 t = median_ms(lambda: csx.compact_config(data, mask, cfg, n=n, out=out, count=count, workspace=ws, host_count=False), args.reps)
                 _count_check(count, k, f"{w} {mname} {fname}")
                 row[f"{fname}_us"] = round(t * 1000, 1)
             row["best"] = min((v, f) for f, v in row.items() if f.endswith("_us") and v is not None)[1][:-3]
             rows.append(row)
-            print(f"{w};{mname};{k};{row['line_ratio']};" + ";".join("" if row[f'{f}_us'] is None else str(row[f'{f}_us']) for f, _, _ in forms)
+            print(f"{w};{mname};{k};{row['line_ratio']};" + ";".join("" if row[f'{f[0]}_us'] is None else str(row[f'{f[0]}_us']) for f in forms)
                   + f";{row['best']}", flush=True)
         del data, out, ws
         torch.cuda.empty_cache()

@@ -649,6 +649,19 @@ static csx_status check_vpt(uint32_t vpt, uint32_t elem_size)
 }
 
 
+// the tile shapes a pipeline is built for: the three-phase writeout takes
+// every tile size and workgroup size, the single launch tiles of 2, 4 or 8
+// vectors per 256-thread workgroup, the others the 8-vector, 256-thread tile;
+// null when (vpt, block) is one of them
+static const char* pipeline_shape_error(int pipeline, uint32_t vpt, uint32_t block)
+{
+    block = block_of(block);
+    if (pipeline == CSX_PIPELINE_THREE_PHASE) return nullptr;
+    if (block != 256) return "only the three-phase writeout takes a workgroup size other than 256";
+    if (pipeline == CSX_PIPELINE_SINGLE) return vpt == 16 ? "the single-launch pipeline takes tiles of 2, 4 or 8 vectors per thread" : nullptr;
+    return vpt != 8 ? "only the three-phase and single-launch pipelines take a tile size other than 8" : nullptr;
+}
+
 static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_out, uint64_t n, uint32_t elem_size, uint32_t chunk_len,
                                int bit_order, int pipeline, uint32_t vpt, uint32_t block, uint64_t* d_count, uint64_t* h_count, void* d_ws,
                                size_t ws_bytes, hipStream_t stream, bool profile = true)
@@ -663,10 +676,8 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
     if (pipeline == CSX_PIPELINE_SINGLE && n > CSX_SINGLE_MAX_ELEMENTS)
         return fail(CSX_ERR_UNSUPPORTED, "the single-launch pipeline takes at most %llu elements", (unsigned long long)CSX_SINGLE_MAX_ELEMENTS);
     if ((s = check_vpt(vpt, elem_size)) || (s = check_block(block))) return s;
-    if (pipeline != CSX_PIPELINE_THREE_PHASE && vpt != 8) return fail(CSX_ERR_INVALID, "only the three-phase writeout takes a tile size other than 8");
+    if (const char* why = pipeline_shape_error(pipeline, vpt, block)) return fail(CSX_ERR_INVALID, "%s", why);
     block = block_of(block);
-    if (pipeline != CSX_PIPELINE_THREE_PHASE && block != 256)
-        return fail(CSX_ERR_INVALID, "only the three-phase writeout takes a workgroup size other than 256");
     const CompactPlan p = plan_compact(n, elem_size, chunk_len);
     if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
     if (n > 0 && (!d_in || !d_mask || !d_out)) return fail(CSX_ERR_INVALID, "null input, mask or output pointer");
@@ -727,7 +738,7 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
         a.n = n;
         a.count_out = count_dst;
         a.hdr = hdr;
-        CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeSingle, true, bit_order, aligned, a, stream); }),
+        CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeSingle, true, bit_order, aligned, a, stream, (int)vpt); }),
                 "csx_compact single launch");
     }
     else if (pipeline == CSX_PIPELINE_TWO_PASS) {
@@ -1462,7 +1473,8 @@ csx_status tune_candidates(const void* in, const uint8_t* mask, void* out, uint6
     cand.push_back(csx_config{CSX_PIPELINE_SPARSE, 1024, 8, 0});
     cand.push_back(csx_config{CSX_PIPELINE_SPARSE, 4096, 8, 0});
     cand.push_back(csx_config{CSX_PIPELINE_FUSED, 1024, 8, 0});
-    if (n <= CSX_SINGLE_MAX_ELEMENTS) cand.push_back(csx_config{CSX_PIPELINE_SINGLE, 1024, 8, 0});
+    if (n <= CSX_SINGLE_MAX_ELEMENTS)
+        for (uint32_t vpt : {2u, 4u, 8u}) cand.push_back(csx_config{CSX_PIPELINE_SINGLE, 1024, vpt, 0});
 
     auto hdr = static_cast<csx::WsHeader*>(ws);
     const uint64_t* count_src = d_count ? d_count : &hdr->count;
@@ -1633,7 +1645,7 @@ csx_status csx_tune_load(const char* path)
             if (check_width(c.w) || check_chunk(c.cfg.chunk_len) || check_vpt(c.cfg.vpt, c.w) || c.log2n > 40 || !(c.sel >= 0 && c.sel <= 1) ||
                 !(c.ms >= 0 && std::isfinite(c.ms)) || (block != 128 && block != 256 && block != 512) ||
                 !known_pipeline(c.cfg.pipeline) || (c.cfg.pipeline == CSX_PIPELINE_SINGLE && c.log2n > 24) ||
-                (c.cfg.pipeline != CSX_PIPELINE_THREE_PHASE && (c.cfg.vpt != 8 || block != 256))) {
+                pipeline_shape_error(c.cfg.pipeline, c.cfg.vpt, block)) {
                 st = fail(CSX_ERR_INVALID, "%s:%u: configuration out of range", path, lineno);
                 break;
             }

@@ -265,7 +265,9 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
 {
     const uint64_t nv = bytes / 16;
     const uint32_t tail = (uint32_t)(bytes % 16);
-    launch_k(k_copy, dim3(grid_for(nv, 256, 1024)), dim3(256), s, static_cast<const u32x4*>(src), static_cast<u32x4*>(dst), nv,
+    // one vector per thread over a grid that covers the copy once (the
+    // fastest plain-copy shape on this HBM); past 2^30 vectors threads loop
+    launch_k(k_copy, dim3(grid_for(nv, 256, 1u << 22)), dim3(256), s, static_cast<const u32x4*>(src), static_cast<u32x4*>(dst), nv,
              static_cast<const uint8_t*>(src) + nv * 16, static_cast<uint8_t*>(dst) + nv * 16, tail);
     return hipGetLastError();
 }

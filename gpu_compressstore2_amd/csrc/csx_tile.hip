@@ -26,6 +26,13 @@ template <int W, int ORDER, bool ALIGNED, int VPT> static void launch_writeout_v
     }
 }
 
+// the single-launch pipeline's tile: 256 threads x VPT vectors
+template <int W, int ORDER, bool ALIGNED, int VPT> static void launch_single_vpt(const TileArgs& a, hipStream_t s)
+{
+    const uint64_t tiles = (a.n + TileCfg<W, VPT>::TE - 1) / TileCfg<W, VPT>::TE;
+    launch_k(k_tile<W, ORDER, ALIGNED, kModeSingle, true, VPT>, dim3((unsigned)tiles), dim3(256), s, a);
+}
+
 template <int W, bool ALIGNED, int VPT> static void launch_pattern_vpt(const TileArgs& a, hipStream_t s)
 {
     const uint64_t tiles = (a.n + TileCfg<W, VPT>::TE - 1) / TileCfg<W, VPT>::TE;
@@ -42,7 +49,11 @@ static void launch_tile_impl(int mode, bool staged, int vpt, int block, const Ti
         launch_k(k_tile<W, ORDER, ALIGNED, kModeFused, true>, dim3(grid), dim3(256), s, a);
     }
     else if (mode == kModeSingle) {
-        launch_k(k_tile<W, ORDER, ALIGNED, kModeSingle, true>, dim3(grid), dim3(256), s, a);
+        switch (vpt) {
+            case 2: launch_single_vpt<W, ORDER, ALIGNED, 2>(a, s); break;
+            case 4: launch_single_vpt<W, ORDER, ALIGNED, 4>(a, s); break;
+            default: launch_single_vpt<W, ORDER, ALIGNED, 8>(a, s); break;
+        }
     }
     else if (mode == kModeGiven) {
         // one tile at a.given_off (inputs of at most one tile: the whole call)
@@ -81,6 +92,7 @@ static void launch_tile_impl(int mode, bool staged, int vpt, int block, const Ti
 template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool aligned, const TileArgs& a, hipStream_t s, int vpt, int block)
 {
     if (vpt != 2 && vpt != 4 && vpt != 8 && !(vpt == 16 && W >= 2)) return hipErrorInvalidValue;
+    if (mode == kModeSingle && vpt == 16) return hipErrorInvalidValue;
     // only the staged writeout's tile takes another workgroup size than 256
     if (block != 128 && block != 256 && block != 512) return hipErrorInvalidValue;
     if (block != 256 && !(mode == kModeWriteout && staged)) return hipErrorInvalidValue;

@@ -287,9 +287,9 @@ uint32_t csx_server_inflight(const csx_server* server);
 /* ---------------------------------------------------------------- tuning
  * A compaction configuration.  pipeline: CSX_PIPELINE_*; chunk_len: the
  * three-phase chunk length; vpt: tile size in 16-byte vectors per thread of
- * the three-phase writeout (2, 4, 8, or 16 when elem_size >= 2; TWO_PASS,
- * FUSED, SPARSE and SINGLE take 8, which they do not use) or of the batched
- * form (2, 4 or 8); count_blocks: grid bound of the batched form's per-tile
+ * the three-phase writeout (2, 4, 8, or 16 when elem_size >= 2), of the
+ * single launch (2, 4 or 8) or of the batched form (2, 4 or 8); TWO_PASS,
+ * FUSED and SPARSE take 8, which they do not use; count_blocks: grid bound of the batched form's per-tile
  * count kernel (0: 4096); block_size: the workgroup size of those tiles
  * (128, 256 or 512; 0 means 256; the other pipelines take 0 or 256) -- the
  * reference's block-size argument (src/kernels/kernel_3pass.cuh:491-533).
@@ -320,7 +320,7 @@ csx_status csx_compact_auto(const void* d_in, const uint8_t* d_mask, void* d_out
  * mask on the device, times every candidate (THREE_PHASE x chunk {256, 1024,
  * 4096} x vpt {2, 4, 8, 16} in 256-thread tiles, THREE_PHASE at chunk 1024 x
  * vpt in 128- and 512-thread tiles, TWO_PASS, FUSED, SPARSE at chunks 1024
- * and 4096, and SINGLE up to its cap) with HIP events -- `reps` calls back to back between two events, three
+ * and 4096, and SINGLE x vpt {2, 4, 8} up to its cap) with HIP events -- `reps` calls back to back between two events, three
  * rounds after one warm-up, the median round's per-call time -- checks that
  * all give the same count,
  * records the fastest in the table and returns it.  Allocates and frees its

@@ -303,6 +303,30 @@ def test_three_phase_workgroup_sizes(O, w, block):
                 assert (got[k:] == canary[k:]).all()
 
 
+@pytest.mark.parametrize("vpt", [2, 4, 8])
+@pytest.mark.parametrize("w", [1, 2, 4, 8])
+def test_single_launch_tile_sizes(O, w, vpt):
+    """The single-launch pipeline with 2-, 4- and 8-vector tiles: just past the
+    one-tile shortcut, a few tiles, and columns whose tiles count more than
+    one round (64 KiB) of mask before them, in both bit orders, with the
+    launch shape it reports."""
+    te = 256 * vpt * (16 // w)
+    one_tile = 256 * 8 * (16 // w)
+    for n, kind in ((one_tile + 1, "bern50"), (max(3 * te + 17, one_tile + 129), "runs"), (600_000 + 333, "bern01"), ((1 << 20) + 77, "bern99")):
+        for order in (oracle.MSB_FIRST, oracle.LSB_FIRST):
+            print(f"single launch vpt={vpt} w={w} n={n} mask={kind} order={order}", flush=True)
+            data = O.fill_values(n, NP[w], seed=n + vpt)
+            mask = dirty_tail(make_mask(O, kind, n, n + vpt, order), n, order)
+            expect, ke = O.compact(data, mask, n, order)
+            canary = np.frombuffer(np.full((n + 16) * w, 0xC3, np.uint8).tobytes(), dtype=NP[w]).copy()
+            d_out = dev(canary)
+            _, k = csx.compact_config(dev(data), dev(mask), csx.Config(csx.PIPELINE_SINGLE, 1024, vpt, 0), n=n, bit_order=order, out=d_out)
+            assert csx.last_launch() == (256, (n + te - 1) // te)
+            got = host(d_out, NP[w])
+            assert k == ke and (got[:k] == expect).all()
+            assert (got[k:] == canary[k:]).all()
+
+
 def test_single_launch_at_its_cap():
     """The single-launch pipeline at CSX_SINGLE_MAX_ELEMENTS (the last tile
     counts 2 MiB of mask before it) against torch.nonzero, a ragged size just

@@ -26,7 +26,8 @@ def candidates(w):
     out.append(csx.Config(csx.PIPELINE_FUSED, 1024, 8, 0))
     out.append(csx.Config(csx.PIPELINE_SPARSE, 1024, 8, 0))
     out.append(csx.Config(csx.PIPELINE_SPARSE, 4096, 8, 0))
-    out.append(csx.Config(csx.PIPELINE_SINGLE, 1024, 8, 0))
+    for vpt in (2, 4, 8):
+        out.append(csx.Config(csx.PIPELINE_SINGLE, 1024, vpt, 0))
     return out
 
 

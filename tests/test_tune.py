@@ -59,6 +59,8 @@ def test_save_load_and_nearest_cell(tmp_path):
         "compact 4 20 0.5 1 1000 8 0.1",  # chunk not a power of two
         "compact 1 20 0.5 1 1024 16 0.1",  # vpt 16 needs 2-byte elements
         "compact 4 20 0.5 2 1024 4 0.1",  # two-pass uses vpt 8
+        "compact 4 20 0.5 4 1024 16 0.1",  # the single launch takes 2, 4 or 8
+        "compact 4 20 0.5 4 1024 4 512 0.1",  # ... in 256-thread tiles
         "compact 4 20 1.5 1 1024 8 0.1",  # selectivity
         "batch 4 16 4096",  # batched vpt
         "hello",
@@ -102,9 +104,18 @@ def test_bad_block_sizes_rejected(tmp_path, line):
         csx.tune_load(str(p))
 
 
+def test_single_launch_rows_take_a_tile_size(tmp_path):
+    p = tmp_path / "t.txt"
+    p.write_text("compact 4 20 0.500000 4 1024 4 256 0.008\ncompact 2 18 0.500000 4 1024 2 0.006\n")
+    csx.tune_load(str(p))
+    c = csx.tune_lookup(4, 1 << 20, 0.5)
+    assert (c.pipeline, c.vpt, c.block_size) == (csx.PIPELINE_SINGLE, 4, 256)
+    assert csx.tune_lookup(2, 1 << 18, 0.5).vpt == 2
+
+
 def test_config_checks_before_device_work():
     L = csx.lib()
-    for pipeline, chunk, vpt, w in [(1, 1024, 3, 4), (1, 1024, 16, 1), (2, 1024, 4, 4), (7, 1024, 8, 4), (1, 100, 8, 4)]:
+    for pipeline, chunk, vpt, w in [(1, 1024, 3, 4), (1, 1024, 16, 1), (2, 1024, 4, 4), (7, 1024, 8, 4), (1, 100, 8, 4), (4, 1024, 16, 4)]:
         c = csx.Config(pipeline, chunk, vpt, 0)
         assert L.csx_compact_config(None, None, None, 10, w, 0, ctypes.byref(c), None, None, None, 0, None) == 1
     assert L.csx_compact_config(None, None, None, 10, 4, 0, None, None, None, None, 0, None) == 1
