@@ -703,8 +703,9 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
         CSX_HIP(csx::launch_lookback_init(hdr, status, 0, stream), "csx_compact init");
         CSX_HIP(csx::launch_store_u64(count_dst, 0, stream), "csx_compact count");
     }
-    else if (n <= tile_elems(elem_size)) {
-        // the whole input is one tile: one workgroup, output offset 0, every pipeline alike
+    else if (n <= 256ull * vpt * (16 / elem_size)) {
+        // the whole input is one tile of the configured size (256 threads x
+        // vpt vectors): one workgroup, output offset 0, every pipeline alike
         csx::TileArgs a{};
         a.in = in;
         a.mask = d_mask;
@@ -712,7 +713,7 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
         a.n = n;
         a.count_out = count_dst;
         a.given_off = 0;
-        CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeGiven, true, bit_order, aligned, a, stream); }),
+        CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeGiven, true, bit_order, aligned, a, stream, (int)vpt); }),
                 "csx_compact single tile");
         if (h_count) return finish_host_count(h_count, count_dst, nullptr, stream);
         return ok();

@@ -56,8 +56,16 @@ static void launch_tile_impl(int mode, bool staged, int vpt, int block, const Ti
         }
     }
     else if (mode == kModeGiven) {
-        // one tile at a.given_off (inputs of at most one tile: the whole call)
-        launch_k(k_tile<W, ORDER, ALIGNED, kModeGiven, true>, dim3(grid), dim3(256), s, a);
+        // one tile of 256 threads x vpt vectors at a.given_off (inputs of at
+        // most one such tile: the whole call)
+        switch (vpt) {
+            case 2: launch_k(k_tile<W, ORDER, ALIGNED, kModeGiven, true, 2>, dim3(1), dim3(256), s, a); break;
+            case 4: launch_k(k_tile<W, ORDER, ALIGNED, kModeGiven, true, 4>, dim3(1), dim3(256), s, a); break;
+            case 16:
+                if constexpr (W >= 2) launch_k(k_tile<W, ORDER, ALIGNED, kModeGiven, true, 16>, dim3(1), dim3(256), s, a);
+                break;
+            default: launch_k(k_tile<W, ORDER, ALIGNED, kModeGiven, true>, dim3(grid), dim3(256), s, a); break;
+        }
     }
     else if (mode == kModePattern) {
         // no mask to order: one instantiation per tile size

@@ -295,8 +295,9 @@ def test_three_phase_workgroup_sizes(O, w, block):
                 d_out = dev(canary)
                 _, k = csx.compact_config(dev(data), dev(mask), csx.Config(csx.PIPELINE_THREE_PHASE, chunk, vpt, 0, block), n=n, bit_order=order,
                                           out=d_out)
-                # the writeout ran block-thread tiles; an input of one default tile runs whole in one 256-thread workgroup
-                one_tile = 256 * 8 * (16 // w)
+                # the writeout ran block-thread tiles; an input of at most one 256-thread tile of
+                # vpt vectors runs whole in one workgroup
+                one_tile = 256 * vpt * (16 // w)
                 assert csx.last_launch() == ((256, 1) if n <= one_tile else (block, (n + te - 1) // te))
                 got = host(d_out, NP[w])
                 assert k == ke and (got[:k] == expect).all()
@@ -306,13 +307,13 @@ def test_three_phase_workgroup_sizes(O, w, block):
 @pytest.mark.parametrize("vpt", [2, 4, 8])
 @pytest.mark.parametrize("w", [1, 2, 4, 8])
 def test_single_launch_tile_sizes(O, w, vpt):
-    """The single-launch pipeline with 2-, 4- and 8-vector tiles: just past the
-    one-tile shortcut, a few tiles, and columns whose tiles count more than
-    one round (64 KiB) of mask before them, in both bit orders, with the
-    launch shape it reports."""
+    """The single-launch pipeline with 2-, 4- and 8-vector tiles: at most one
+    tile (one workgroup), just past it, a few tiles, and columns whose tiles
+    count more than one round (64 KiB) of mask before them, in both bit
+    orders, with the launch shape it reports."""
     te = 256 * vpt * (16 // w)
-    one_tile = 256 * 8 * (16 // w)
-    for n, kind in ((one_tile + 1, "bern50"), (max(3 * te + 17, one_tile + 129), "runs"), (600_000 + 333, "bern01"), ((1 << 20) + 77, "bern99")):
+    for n, kind in ((te - 5, "bern50"), (te, "runs"), (te + 1, "bern50"), (3 * te + 17, "runs"), (600_000 + 333, "bern01"),
+                    ((1 << 20) + 77, "bern99")):
         for order in (oracle.MSB_FIRST, oracle.LSB_FIRST):
             print(f"single launch vpt={vpt} w={w} n={n} mask={kind} order={order}", flush=True)
             data = O.fill_values(n, NP[w], seed=n + vpt)
@@ -321,7 +322,7 @@ def test_single_launch_tile_sizes(O, w, vpt):
             canary = np.frombuffer(np.full((n + 16) * w, 0xC3, np.uint8).tobytes(), dtype=NP[w]).copy()
             d_out = dev(canary)
             _, k = csx.compact_config(dev(data), dev(mask), csx.Config(csx.PIPELINE_SINGLE, 1024, vpt, 0), n=n, bit_order=order, out=d_out)
-            assert csx.last_launch() == (256, (n + te - 1) // te)
+            assert csx.last_launch() == (256, (n + te - 1) // te)  # up to one tile: the one-workgroup form
             got = host(d_out, NP[w])
             assert k == ke and (got[:k] == expect).all()
             assert (got[k:] == canary[k:]).all()
