@@ -107,18 +107,17 @@ hipError_t launch_popc_blocks(const uint8_t* mask, uint64_t n, uint32_t log2b, u
     if (blocks == 0) return hipSuccess;
     const bool aligned = ((uintptr_t)mask & 15) == 0;
     const unsigned g = (unsigned)blocks;
-    if (order == kBitMsb) {
-        if (aligned)
-            k_popc_blocks<kBitMsb, true><<<g, kPopbThreads, 0, s>>>(mask, n, log2b, log2u, uoffs, btotals, hdr);
-        else
-            k_popc_blocks<kBitMsb, false><<<g, kPopbThreads, 0, s>>>(mask, n, log2b, log2u, uoffs, btotals, hdr);
-    }
-    else {
-        if (aligned)
-            k_popc_blocks<kBitLsb, true><<<g, kPopbThreads, 0, s>>>(mask, n, log2b, log2u, uoffs, btotals, hdr);
-        else
-            k_popc_blocks<kBitLsb, false><<<g, kPopbThreads, 0, s>>>(mask, n, log2b, log2u, uoffs, btotals, hdr);
-    }
+    // workgroups of 256, 512 or 1024 threads: a block of 2^17, 2^18 or at
+    // least 2^19 elements is one round
+    const uint32_t grow = log2b - kPopbMinLog2 < 2 ? log2b - kPopbMinLog2 : 2;
+    const unsigned threads = (unsigned)kPopbThreads << grow;
+    using K = void (*)(const uint8_t*, uint64_t, uint32_t, uint32_t, uint32_t*, uint32_t*, WsHeader*);
+    static const K kernels[2][2][3] = {
+        {{k_popc_blocks<kBitLsb, false, 256>, k_popc_blocks<kBitLsb, false, 512>, k_popc_blocks<kBitLsb, false, 1024>},
+         {k_popc_blocks<kBitLsb, true, 256>, k_popc_blocks<kBitLsb, true, 512>, k_popc_blocks<kBitLsb, true, 1024>}},
+        {{k_popc_blocks<kBitMsb, false, 256>, k_popc_blocks<kBitMsb, false, 512>, k_popc_blocks<kBitMsb, false, 1024>},
+         {k_popc_blocks<kBitMsb, true, 256>, k_popc_blocks<kBitMsb, true, 512>, k_popc_blocks<kBitMsb, true, 1024>}}};
+    kernels[order == kBitMsb][aligned][grow]<<<g, threads, 0, s>>>(mask, n, log2b, log2u, uoffs, btotals, hdr);
     return hipGetLastError();
 }
 

@@ -200,10 +200,9 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const uint32_t* counts, Out
 // ======================================================================
 //
 // K1 (k_popc_blocks).  The column is cut into blocks of BE = 2^log2b
-// elements (2^17 up to 2^32 elements, more beyond so that there are at most
-// kPopbMaxBlocks), one 256-thread workgroup each, taken in rounds of 2^17
-// elements = 1024 mask vectors (16 bytes, 128 elements each).  In a round,
-// wave w owns vectors [256 w, 256 w + 256): its load j reads vector
+// elements (popb_log2), one workgroup of THREADS threads each, taken in
+// rounds of THREADS * 4 mask vectors (16 bytes, 128 elements each).  In a
+// round, wave w owns vectors [256 w, 256 w + 256): its load j reads vector
 // 256 w + 64 j + lane, so every wave load reads 1 KiB contiguous, and the
 // four loads of the next round are issued before the current round is
 // processed.  A DPP scan of each load's popcounts, the four loads' totals
@@ -217,19 +216,26 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const uint32_t* counts, Out
 //
 // No workgroup waits for another, so there is no look-back state to
 // initialise; a writeout tile's output offset is bprefix[its block] +
-// uoffs[its first unit], two words it requests before its mask.  Bits at
+// uoffs[its first unit], two words every wave requests before its mask
+// (scalar loads with no branch around them, so no wait for them comes
+// before the mask load, and no barrier shares them).  Bits at
 // element index >= n do not count, whatever the buffer holds.
 constexpr int kPopbThreads = 256;
 constexpr int kPopbLoads = 4;  // vectors per thread and round
-constexpr int kPopbWaves = kPopbThreads / kWave;
 static_assert((1u << kPopbMinLog2) == kPopbThreads * kPopbLoads * 128, "a round is one 2^17-element block");
 
 // `hdr` (nullable): its error word is cleared, so that the workspace error
 // query answers for this call too (no kernel of these pipelines sets it).
-template <int ORDER, bool ALIGNED>
-__global__ __launch_bounds__(kPopbThreads) void k_popc_blocks(const uint8_t* __restrict__ mask, uint64_t n, uint32_t log2b, uint32_t log2u,
-                                                              uint32_t* __restrict__ uoffs, uint32_t* __restrict__ btotals, WsHeader* hdr)
+// THREADS = 256, 512 for blocks of 2^18 elements, 1024 for 2^19 and more: a
+// block of up to 2^19 elements is one round, with the same loads in flight
+// as 256-thread workgroups over 2^17-element blocks (whose larger blocks
+// would take their rounds one after another).
+template <int ORDER, bool ALIGNED, int THREADS = kPopbThreads>
+__global__ __launch_bounds__(THREADS) void k_popc_blocks(const uint8_t* __restrict__ mask, uint64_t n, uint32_t log2b, uint32_t log2u,
+                                                         uint32_t* __restrict__ uoffs, uint32_t* __restrict__ btotals, WsHeader* hdr)
 {
+    constexpr int kPopbWaves = THREADS / kWave;
+    constexpr uint32_t kRoundVecs = THREADS * kPopbLoads;  // 16-byte mask vectors per round
     __shared__ uint32_t s_wsum[2][kPopbWaves];
     if (hdr && blockIdx.x == 0 && threadIdx.x == 0) hdr->error = 0;
     const uint32_t t = threadIdx.x, lane = lane_id(), wid = t / kWave;
@@ -237,11 +243,11 @@ __global__ __launch_bounds__(kPopbThreads) void k_popc_blocks(const uint8_t* __r
     const uint64_t e_block = (uint64_t)blockIdx.x << log2b;
     const uint64_t e_end = min(n, e_block + (1ull << log2b));
     const uint64_t nv = (e_end - e_block + 127) / 128;  // vectors holding elements of this block
-    const uint32_t rounds = (uint32_t)((nv + 1023) / 1024);
+    const uint32_t rounds = (uint32_t)((nv + kRoundVecs - 1) / kRoundVecs);
     const uint8_t* bmask = mask + e_block / 8;
     const uint64_t bbytes = nbytes - e_block / 8;
     // vector of load j in round r (block-relative)
-    auto vec_of = [&](uint32_t r, int j) { return (uint64_t)r * 1024 + wid * 256 + j * kWave + lane; };
+    auto vec_of = [&](uint32_t r, int j) { return (uint64_t)r * kRoundVecs + wid * 256 + j * kWave + lane; };
     u32x4 x[kPopbLoads];
 #pragma unroll
     for (int j = 0; j < kPopbLoads; j++) x[j] = vec_of(0, j) < nv ? load_mask16<ALIGNED>(bmask, vec_of(0, j) * 16, bbytes) : u32x4{0, 0, 0, 0};
@@ -362,6 +368,10 @@ template <typename C> struct TileShared {
     uint64_t tile;
     uint64_t off;
 };
+
+// what an absent offset array reads (zeros, never written; a global-memory
+// variable so that the loads stay global, not flat, loads)
+static __device__ uint64_t kZeros[2] __attribute__((aligned(16))) = {0, 0};
 
 template <int W> __device__ __forceinline__ uint64_t vec_elem(const u32x4& x, int k)
 {
@@ -503,14 +513,15 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
     if constexpr (MODE == kModeSingle) prefix_start<C, ALIGNED>(a, tbase, pfx);
 
     // three-phase writeout with two-level offsets: the tile's block prefix
-    // and its unit's offset, requested ahead of the mask
+    // and its unit's offset, requested ahead of the mask.  The addresses are
+    // chosen without a branch and loaded by every wave (uniform: scalar
+    // loads), so that no wait for them lands before the mask load and no
+    // barrier is needed to share them; an absent array reads kZeros.
     uint64_t bpre = 0;
     uint32_t uoff = 0;
     if constexpr (MODE == kModeWriteout) {
-        if (a.bprefix && t == 0) {
-            bpre = a.bprefix[tbase >> a.log2b];
-            uoff = a.uoffs[tbase >> a.log2c];
-        }
+        bpre = *(a.bprefix ? a.bprefix + (tbase >> a.log2b) : kZeros);
+        uoff = *(a.uoffs ? a.uoffs + (tbase >> a.log2c) : reinterpret_cast<const uint32_t*>(kZeros));
     }
 
     // ---- mask words, word prefixes, tile count ----
@@ -705,8 +716,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         __syncthreads();
     }
     else if (a.bprefix) {
-        if (t == 0) sh.off = bpre + uoff;
-        __syncthreads();
+        // every thread holds bpre and uoff: no barrier here
     }
     else {
         const uint64_t chunk = tbase >> a.log2c;
@@ -728,7 +738,8 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         if (t == 0) sh.off = (a.offsets ? a.offsets[chunk] : (uint64_t)a.offsets32[chunk]) + extra;
         __syncthreads();
     }
-    const uint64_t toff = sh.off + (a.out_base ? *a.out_base : 0ull);
+    const uint64_t tile_off = MODE == kModeWriteout && a.bprefix ? bpre + uoff : sh.off;
+    const uint64_t toff = tile_off + (a.out_base ? *a.out_base : 0ull);
 
     // ---- ranks, then staging (or direct stores) ----
     const uintptr_t gstart = (uintptr_t)(a.out + toff * W);
