@@ -950,13 +950,10 @@ __global__ __launch_bounds__(256) void k_sparse(TileArgs a)
         const uint64_t e0 = step * kSparseStep;
         const uint64_t el = e0 + 128ull * lane;
         // two-level offsets: the step's block prefix and unit offset,
-        // requested ahead of the mask
-        uint64_t bpre = 0;
-        uint32_t uoff = 0;
-        if (a.bprefix) {
-            bpre = a.bprefix[e0 >> a.log2b];
-            uoff = a.uoffs[e0 >> a.log2c];
-        }
+        // requested ahead of the mask with no branch around them (an absent
+        // array reads kZeros), so that no wait for them precedes the mask load
+        const uint64_t bpre = *(a.bprefix ? a.bprefix + (e0 >> a.log2b) : kZeros);
+        const uint32_t uoff = *(a.uoffs ? a.uoffs + (e0 >> a.log2c) : reinterpret_cast<const uint32_t*>(kZeros));
         uint32_t w[4] = {0, 0, 0, 0};
         if (el < n) {
             const u32x4 x = load_mask16<ALIGNED>(a.mask, el / 8, nbytes);
