@@ -606,10 +606,39 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
             run += popc32(w[k]);
         }
     }
+    // This thread's mask words and their prefixes, read together (one LDS
+    // wait) once they are visible: the selected bits of each of its vectors
+    // and the tile rank of each vector's first element.  Where one barrier
+    // publishes words and count, they are read with the count, before the
+    // empty-tile test, so that the data loads below issue back to back.
+    // (The single launch, whose prefix loads hold 64 registers, reads its
+    // words after the empty-tile test and the prefixes at the ranks, as
+    // registers held across the test would cost it a wave per SIMD.)
+    constexpr bool kEarlyRank = MODE != kModeSingle;
+    bool words_read = false;
+    uint32_t bits[C::VPT], rank0[C::VPT];
+    auto read_words = [&] {
+#pragma unroll
+        for (int j = 0; j < C::VPT; j++) {
+            const uint32_t e0 = (j * C::BLOCK + t) * C::EPV;
+            const uint32_t word = sh.words[e0 >> 5];
+            bits[j] = (word >> (e0 & 31)) & (uint32_t)((1ull << C::EPV) - 1);
+            rank0[j] = kEarlyRank ? sh.wpre[e0 >> 5] + popc32(low_bits(word, e0 & 31)) : 0u;
+        }
+    };
     if constexpr (MODE != kModeTwoPass && C::MASK_THREADS <= kWave) {
         __syncthreads();
         tile_count = sh.count;
         published = true;
+        if constexpr (kEarlyRank) {
+            read_words();
+            words_read = true;
+            // an empty asm statement taking the values: the compiler would
+            // otherwise sink the reads below the empty-tile branch, behind a
+            // wait for the count alone
+#pragma unroll
+            for (int j = 0; j < C::VPT; j++) asm volatile("" ::"v"(bits[j]), "v"(rank0[j]));
+        }
     }
     if constexpr (MODE != kModeFused) {
         // an empty tile has nothing to read or write (the fused form must
@@ -628,16 +657,15 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         }
     }
     if (!published) __syncthreads();
+    if (!words_read) read_words();
 
     // ---- data loads (predicated on the vector's mask bits) ----
     u32x4 vec[C::VPT];
-    uint32_t bits[C::VPT];
     const uint8_t* tin = a.in + tbase * W;
 #pragma unroll
     for (int j = 0; j < C::VPT; j++) {
         const uint32_t v = j * C::BLOCK + t;
         const uint32_t e0 = v * C::EPV;
-        bits[j] = (sh.words[e0 >> 5] >> (e0 & 31)) & (uint32_t)((1ull << C::EPV) - 1);
         vec[j] = u32x4{0, 0, 0, 0};
         if (bits[j]) {
             if (ALIGNED && !partial) {
@@ -747,10 +775,11 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
 #pragma unroll
     for (int j = 0; j < C::VPT; j++) {
         if (!bits[j]) continue;
-        const uint32_t v = j * C::BLOCK + t;
-        const uint32_t e0 = v * C::EPV;
-        const uint32_t word = e0 >> 5;
-        uint32_t r = sh.wpre[word] + popc32(low_bits(sh.words[word], e0 & 31));
+        uint32_t r = rank0[j];
+        if constexpr (!kEarlyRank) {
+            const uint32_t e0 = (j * C::BLOCK + t) * C::EPV;
+            r = sh.wpre[e0 >> 5] + popc32(low_bits(sh.words[e0 >> 5], e0 & 31));
+        }
 #pragma unroll
         for (int k = 0; k < C::EPV; k++) {
             if ((bits[j] >> k) & 1) {
