@@ -243,7 +243,10 @@ __global__ __launch_bounds__(THREADS) void k_popc_blocks(const uint8_t* __restri
     const uint64_t e_block = (uint64_t)blockIdx.x << log2b;
     const uint64_t e_end = min(n, e_block + (1ull << log2b));
     const uint64_t nv = (e_end - e_block + 127) / 128;  // vectors holding elements of this block
-    const uint32_t rounds = (uint32_t)((nv + kRoundVecs - 1) / kRoundVecs);
+    // 256 and 512 threads are launched for blocks of exactly one round (2^17
+    // and 2^18 elements): a constant round count drops the next round's
+    // prefetch registers, so 8 workgroups of 256 fit per CU in both bit orders
+    const uint32_t rounds = THREADS < 1024 ? 1u : (uint32_t)((nv + kRoundVecs - 1) / kRoundVecs);
     const uint8_t* bmask = mask + e_block / 8;
     const uint64_t bbytes = nbytes - e_block / 8;
     // vector of load j in round r (block-relative)
