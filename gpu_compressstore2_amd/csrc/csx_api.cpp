@@ -1348,14 +1348,6 @@ csx_status csx_compact_predicate(const void* d_in, uint64_t n, const csx_predica
     else {
         CSX_HIP(csx::launch_predicate(d_in, n, p, mask, bit_order, counts, nullptr, hdr, status, cp.scan_tiles, stream), "csx_compact_predicate mask");
         CSX_HIP(csx::launch_scan(counts, offsets, cp.chunks, count_dst, hdr, status, stream, true), "csx_compact_predicate scan");
-        csx::TileArgs a{};
-        a.in = static_cast<const uint8_t*>(d_in);
-        a.mask = mask;
-        a.out = static_cast<uint8_t*>(d_out);
-        a.n = n;
-        a.offsets = offsets;
-        a.counts = counts;
-        a.log2c = log2u(kPredChunk);
         const bool aligned = (((uintptr_t)d_in | (uintptr_t)mask) & 15) == 0;
         // the writeout's tile size: the tuned three-phase choice for this width
         // and size at unknown selectivity (the chunk offsets serve any tile size)
@@ -1365,6 +1357,23 @@ csx_status csx_compact_predicate(const void* d_in, uint64_t n, const csx_predica
         if (tc.pipeline == CSX_PIPELINE_THREE_PHASE && launch_fits((n + tte - 1) / tte)) {
             vpt = tc.vpt;
             block = block_of(tc.block_size);
+        }
+        csx::TileArgs a{};
+        a.in = static_cast<const uint8_t*>(d_in);
+        a.mask = mask;
+        a.out = static_cast<uint8_t*>(d_out);
+        a.n = n;
+        if ((uint64_t)block * vpt * (16 / w) % kPredChunk == 0) {
+            // every tile starts a chunk, so the chunk's exclusive offset is the
+            // tile's: the writeout takes it as its block prefix (a scalar load
+            // ahead of the mask, no barrier), with no unit offset
+            a.bprefix = offsets;
+            a.log2b = log2u(kPredChunk);
+        }
+        else {
+            a.offsets = offsets;
+            a.counts = counts;
+            a.log2c = log2u(kPredChunk);
         }
         CSX_HIP(timed(stream, [&] { return dispatch_tile(w, csx::kModeWriteout, true, bit_order, aligned, a, stream, (int)vpt, (int)block); }),
                 "csx_compact_predicate writeout");
