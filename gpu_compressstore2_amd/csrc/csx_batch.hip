@@ -8,7 +8,7 @@
 //              count, zeroed per-segment counts and look-back state
 //   K1 counts  persistent grid walking the (segment, tile) queue, one wave
 //              load per tile (per group of tiles when a tile has fewer than
-//              64 mask vectors): selected count of every tile
+//              64 mask vectors): selected count and segment of every tile
 //   K2 scan    the look-back scan over all tile counts (length from device)
 //   K3 write   one workgroup per (segment, tile): the tile engine with the
 //              tile's offset made local to its segment; the last tile of a
@@ -101,7 +101,8 @@ constexpr int kCountUnroll = 4;
 // past that segment's end finds its own segment (rare: a column end inside a
 // load).
 template <int W, int ORDER, uint32_t VEC, uint64_t TE>
-__device__ __forceinline__ void batch_counts_packed(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, uint32_t* tcounts)
+__device__ __forceinline__ void batch_counts_packed(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, uint32_t* tcounts,
+                                                    uint32_t* tseg)
 {
     constexpr uint32_t TPW = kWave / VEC;
     static_assert(VEC * TPW == kWave && TE == 128ull * VEC, "tiles pack a wave load exactly");
@@ -139,6 +140,7 @@ __device__ __forceinline__ void batch_counts_packed(const Segment* segs, uint32_
             uint64_t lf = s_first, le = s_end;
             const uint8_t* lmask = seg.mask;
             uint64_t ln = seg.n;
+            uint32_t lseg = s;
             if (live[u] && gu >= le) {
                 // gu < total = tile_prefix[nseg]: the walk stops before nseg
                 uint32_t ls = s;
@@ -149,7 +151,10 @@ __device__ __forceinline__ void batch_counts_packed(const Segment* segs, uint32_
                 }
                 lmask = segs[ls].mask;
                 ln = segs[ls].n;
+                lseg = ls;
             }
+            // the tile's segment, for the write kernels (with the count below)
+            if (live[u] && v == 0) tseg[gu] = lseg;
             if (live[u]) {
                 const uint64_t lt = gu - lf;
                 const uint64_t nbytes = (ln + 7) / 8;
@@ -181,12 +186,13 @@ __device__ __forceinline__ void batch_counts_packed(const Segment* segs, uint32_
 }
 
 template <int W, int ORDER, int VPT, int BLOCK>
-__global__ __launch_bounds__(256) void k_batch_counts(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, uint32_t* tcounts)
+__global__ __launch_bounds__(256) void k_batch_counts(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, uint32_t* tcounts,
+                                                      uint32_t* tseg)
 {
     constexpr uint64_t TE = TileCfg<W, VPT, BLOCK>::TE;
     constexpr uint32_t VEC_PER_TILE = (uint32_t)(TE / 128);
     if constexpr (VEC_PER_TILE < kWave) {
-        batch_counts_packed<W, ORDER, VEC_PER_TILE, TE>(segs, nseg, tile_prefix, tcounts);
+        batch_counts_packed<W, ORDER, VEC_PER_TILE, TE>(segs, nseg, tile_prefix, tcounts, tseg);
         return;
     }
     constexpr uint32_t VPL = (VEC_PER_TILE + kWave - 1) / kWave;  // mask vectors per lane per tile
@@ -223,6 +229,7 @@ __global__ __launch_bounds__(256) void k_batch_counts(const Segment* segs, uint3
             const bool aligned = ((uintptr_t)seg.mask & 15) == 0;
             n_of[u] = seg.n;
             e_of[u] = lt * TE;
+            if (lane == 0) tseg[gu] = s;  // the tile's segment, for the write kernels
 #pragma unroll
             for (uint32_t q = 0; q < VPL; q++) {
                 const uint32_t v = q * kWave + lane;
@@ -267,46 +274,50 @@ constexpr uint64_t kSparseBlocks = 2048;
 
 template <int W, int ORDER, int VPT, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_batch_write(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, const uint64_t* goff,
-                                                       const uint32_t* tcounts, uint64_t* counts)
+                                                       const uint32_t* tcounts, const uint32_t* tseg, uint64_t* counts)
 {
     using C = TileCfg<W, VPT, BLOCK>;
     __shared__ TileShared<C> sh;
-    __shared__ uint32_t s_seg;
-    __shared__ uint32_t s_live;
     // runs of consecutive tiles per XCD (see xcd_runs)
     const uint64_t g = xcd_runs(blockIdx.x, gridDim.x);
-    // The real tile count, the segment search, the tile's global offset and
-    // its count are requested together (the search reads only prefixes below
-    // nseg, and g is below the grid, the max_tiles words of goff and
-    // tcounts), so the tile waits one round trip less before its mask load; a
-    // surplus tile of the host's bound leaves after the barrier.
-    if (threadIdx.x < kWave) {
-        const uint64_t total = tile_prefix[nseg];
-        const uint32_t s0 = find_segment(tile_prefix, nseg, g);  // wave 0 together
-        if (threadIdx.x == 0) {
-            s_seg = s0;
-            s_live = g < total;
-        }
-    }
+    // The real tile count and the tile's segment (written by the count
+    // kernel), global offset and count are requested together, wave-uniform
+    // scalar loads (g is below the grid, the max_tiles words of tseg, goff
+    // and tcounts); then the segment's first tile and descriptor: two round
+    // trips before the mask load, with no search and no barrier.  A surplus
+    // tile of the host's bound has an unwritten segment word: clamped, its
+    // loads stay in bounds, and it leaves once the total is in.  The empty
+    // asm statements hold each round's values before the next, so the
+    // compiler issues a round's loads together instead of one after another.
+    const uint64_t total = tile_prefix[nseg];
+    const uint32_t sraw = tseg[g];
     const uint64_t gofs = goff[g];
     const uint32_t tc = tcounts[g];
-    __syncthreads();
-    if (!s_live) return;  // uniform
-    const uint32_t s = s_seg;
-    const uint64_t first = tile_prefix[s];
+    asm volatile("" ::"s"(total), "s"(sraw), "s"(gofs), "s"(tc));
+    const uint32_t s = (uint32_t)__builtin_amdgcn_readfirstlane((int)(sraw < nseg ? sraw : nseg - 1));
+    const uint64_t first = load_invariant(tile_prefix + s);
+    Segment seg;  // field by field: a whole-struct copy is not made a scalar load
+    seg.in = load_invariant(&segs[s].in);
+    seg.mask = load_invariant(&segs[s].mask);
+    seg.out = load_invariant(&segs[s].out);
+    seg.n = load_invariant(&segs[s].n);
+    asm volatile("" ::"s"(first), "s"(seg.in), "s"(seg.mask), "s"(seg.out), "s"(seg.n));
+    // both below 2^32 (the grid is one launch; the total is clamped to it):
+    // a 32-bit test stays scalar, where a 64-bit one would move the total to
+    // a vector register and wait for it ahead of the other loads
+    if ((uint32_t)g >= (uint32_t)total) return;  // uniform
     if (tc <= C::TE / kSparseDensity) {
         // empty, or sparse and written by k_batch_sparse: only the segment's
         // last tile has something to do here, the segment's count
         if (threadIdx.x == 0 && g + 1 == tile_prefix[s + 1]) counts[s] = gofs - goff[first] + tc;
         return;
     }
-    const Segment seg = segs[s];
     TileArgs a{};
     a.in = static_cast<const uint8_t*>(seg.in);
     a.mask = seg.mask;
     a.out = static_cast<uint8_t*>(seg.out);
     a.n = seg.n;
-    a.given_off = gofs - goff[first];
+    a.given_off = gofs - load_invariant(goff + first);  // scalar: the mask's wait does not include it
     a.count_out = counts + s;
     if ((((uintptr_t)a.in) | ((uintptr_t)a.mask)) & 15)
         tile_body<W, ORDER, false, kModeGiven, true, VPT, BLOCK>(a, sh, g - first);
@@ -319,14 +330,14 @@ __global__ __launch_bounds__(BLOCK) void k_batch_write(const Segment* segs, uint
 // of tile base + l W, base = w, w + 64 W, ... -- so that the consecutive
 // sparse tiles of one column spread over many waves instead of queueing in
 // one.  Each sparse tile in the wave's ballot is written in turn: its segment
-// from the wave-cooperative search, then a lane per 16-byte mask vector (128
+// from the lane that read its count, then a lane per 16-byte mask vector (128
 // elements) of the tile, ranks from a wave scan, the selected elements
 // gathered (up to four loads in flight) and stored one by one -- no LDS, so
 // the tile costs a few wave instructions instead of a workgroup of the tile
 // engine waiting on its loads.
 template <int W, int ORDER, int VPT, int BLOCK>
 __global__ __launch_bounds__(256) void k_batch_sparse(const Segment* segs, uint32_t nseg, const uint64_t* tile_prefix, const uint32_t* tcounts,
-                                                      const uint64_t* goff)
+                                                      const uint32_t* tseg, const uint64_t* goff)
 {
     constexpr uint64_t TE = TileCfg<W, VPT, BLOCK>::TE;
     constexpr uint32_t VEC = (uint32_t)(TE / 128);
@@ -338,11 +349,13 @@ __global__ __launch_bounds__(256) void k_batch_sparse(const Segment* segs, uint3
     for (uint64_t base = wave; base < total; base += kWave * waves) {
         const uint64_t gl = base + lane * waves;
         const uint32_t tc = gl < total ? tcounts[gl] : 0u;
+        const uint32_t sl = gl < total ? tseg[gl] : 0u;  // the lane's tile's segment
         uint64_t todo = __ballot(tc >= 1 && tc <= THR);
         while (todo) {
-            const uint64_t g = base + (uint64_t)__builtin_ctzll(todo) * waves;
+            const uint32_t idx = (uint32_t)__builtin_ctzll(todo);
+            const uint64_t g = base + (uint64_t)idx * waves;
             todo &= todo - 1;
-            const uint32_t s = find_segment(tile_prefix, nseg, g);
+            const uint32_t s = (uint32_t)__builtin_amdgcn_readlane((int)sl, (int)idx);
             const uint64_t s_first = tile_prefix[s];
             const Segment seg = segs[s];
             const uint64_t n = seg.n, nbytes = (n + 7) / 8;
@@ -414,7 +427,7 @@ uint64_t batch_max_tiles(uint32_t w, uint32_t nseg, uint64_t max_total, uint32_t
 
 template <int W, int ORDER, int VPT, int BLOCK>
 static hipError_t batch_impl(const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix, uint32_t* tcounts,
-                             uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks, hipStream_t s)
+                             uint32_t* tseg, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks, hipStream_t s)
 {
     const uint64_t status_words = (max_tiles + kScanTile - 1) / kScanTile;
     k_batch_plan<<<1, 256, 0, s>>>(segs, nseg, TileCfg<W, VPT, BLOCK>::TE, max_tiles, tile_prefix, counts, hdr, status, status_words);
@@ -422,65 +435,65 @@ static hipError_t batch_impl(const Segment* segs, uint32_t nseg, uint64_t max_ti
     if (e != hipSuccess || max_tiles == 0) return e;
     uint64_t g1 = (max_tiles + 3) / 4;
     if (g1 > count_blocks) g1 = count_blocks;
-    k_batch_counts<W, ORDER, VPT, BLOCK><<<(unsigned)g1, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts);
+    k_batch_counts<W, ORDER, VPT, BLOCK><<<(unsigned)g1, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts, tseg);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_scan<uint64_t><<<(unsigned)status_words, kScanBlock, 0, s>>>(tcounts, goff, max_tiles, nullptr, hdr, status, tile_prefix + nseg);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_batch_write<W, ORDER, VPT, BLOCK><<<(unsigned)max_tiles, BLOCK, 0, s>>>(segs, nseg, tile_prefix, goff, tcounts, counts);
+    k_batch_write<W, ORDER, VPT, BLOCK><<<(unsigned)max_tiles, BLOCK, 0, s>>>(segs, nseg, tile_prefix, goff, tcounts, tseg, counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // the sparse tiles: a persistent grid resident at once (8 workgroups per
     // CU; with the count kernel's larger grid the waves queue in rounds and a
     // batch without sparse tiles took 1 % longer)
     const uint64_t g4 = g1 < kSparseBlocks ? g1 : kSparseBlocks;
-    k_batch_sparse<W, ORDER, VPT, BLOCK><<<(unsigned)g4, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts, goff);
+    k_batch_sparse<W, ORDER, VPT, BLOCK><<<(unsigned)g4, 256, 0, s>>>(segs, nseg, tile_prefix, tcounts, tseg, goff);
     return hipGetLastError();
 }
 
 template <int W, int VPT, int BLOCK>
 static hipError_t batch_order(int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
-                              uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks, hipStream_t s)
+                              uint32_t* tcounts, uint32_t* tseg, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks, hipStream_t s)
 {
     return order == kBitMsb
-               ? batch_impl<W, kBitMsb, VPT, BLOCK>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s)
-               : batch_impl<W, kBitLsb, VPT, BLOCK>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+               ? batch_impl<W, kBitMsb, VPT, BLOCK>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s)
+               : batch_impl<W, kBitLsb, VPT, BLOCK>(segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
 }
 
 template <int W, int VPT>
 static hipError_t batch_block(uint32_t block, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts,
-                              uint64_t* tile_prefix, uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks,
+                              uint64_t* tile_prefix, uint32_t* tcounts, uint32_t* tseg, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks,
                               hipStream_t s)
 {
     switch (block) {
-        case 128: return batch_order<W, VPT, 128>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
-        case 256: return batch_order<W, VPT, 256>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
-        case 512: return batch_order<W, VPT, 512>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 128: return batch_order<W, VPT, 128>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
+        case 256: return batch_order<W, VPT, 256>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
+        case 512: return batch_order<W, VPT, 512>(order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <int W>
 static hipError_t batch_vpt(uint32_t vpt, uint32_t block, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts,
-                            uint64_t* tile_prefix, uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks,
+                            uint64_t* tile_prefix, uint32_t* tcounts, uint32_t* tseg, uint64_t* goff, WsHeader* hdr, uint64_t* status, uint32_t count_blocks,
                             hipStream_t s)
 {
     switch (vpt) {
-        case 2: return batch_block<W, 2>(block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
-        case 4: return batch_block<W, 4>(block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
-        case 8: return batch_block<W, 8>(block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 2: return batch_block<W, 2>(block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
+        case 4: return batch_block<W, 4>(block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
+        case 8: return batch_block<W, 8>(block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_batch(uint32_t w, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
-                        uint32_t* tcounts, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s, uint32_t vpt, uint32_t count_blocks,
+                        uint32_t* tcounts, uint32_t* tseg, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s, uint32_t vpt, uint32_t count_blocks,
                         uint32_t block)
 {
     if (count_blocks < 1) count_blocks = 1;
     switch (w) {
-        case 1: return batch_vpt<1>(vpt, block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
-        case 2: return batch_vpt<2>(vpt, block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
-        case 4: return batch_vpt<4>(vpt, block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
-        default: return batch_vpt<8>(vpt, block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, goff, hdr, status, count_blocks, s);
+        case 1: return batch_vpt<1>(vpt, block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
+        case 2: return batch_vpt<2>(vpt, block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
+        case 4: return batch_vpt<4>(vpt, block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
+        default: return batch_vpt<8>(vpt, block, order, segs, nseg, max_tiles, counts, tile_prefix, tcounts, tseg, goff, hdr, status, count_blocks, s);
     }
 }
 

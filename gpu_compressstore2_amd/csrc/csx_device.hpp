@@ -229,6 +229,14 @@ __device__ __forceinline__ uint64_t xcd_runs(uint64_t b, uint64_t n)
 
 // ---- misc ---------------------------------------------------------------
 
+// A value that no workgroup of the running launch writes, read through the
+// constant address space: at a wave-uniform address it is a scalar load
+// (into SGPRs), whatever the kernel does before it.
+template <typename T> __device__ __forceinline__ T load_invariant(const T* p)
+{
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+
 // store 32 mask bits (LSB-first word of elements 32*wi ..) in the caller's order,
 // without touching bytes at or past nbytes
 template <int ORDER> __device__ __forceinline__ void store_mask_word(uint8_t* mask, uint64_t wi, uint32_t word, uint64_t nbytes)
