@@ -698,26 +698,26 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
     }
 
     // ---- tile output offset ----
+    // Two-pass (the block scan's total), given, pattern (arithmetic on the
+    // tile index) and single launch (prefix_finish's result): every thread
+    // holds the offset, so no barrier shares it.
+    constexpr bool kOwnOffset = MODE == kModeTwoPass || MODE == kModeGiven || MODE == kModePattern || MODE == kModeSingle;
+    uint64_t own_off = 0;
     if constexpr (MODE == kModeTwoPass) {
-        if (t == 0) {
-            sh.off = prefix_total;
-            if (tile == ntiles - 1 && a.count_out) *a.count_out = prefix_total + tile_count;
-        }
-        __syncthreads();
+        own_off = prefix_total;
     }
     else if constexpr (MODE == kModeGiven) {
-        if (t == 0) {
-            sh.off = a.given_off;
-            if (tile == ntiles - 1 && a.count_out) *a.count_out = a.given_off + tile_count;
-        }
-        __syncthreads();
+        own_off = a.given_off;
     }
     else if constexpr (MODE == kModePattern) {
-        if (t == 0) {
-            sh.off = pattern_offset<C::TE>(a, tile);
-            if (tile == ntiles - 1 && a.count_out) *a.count_out = sh.off + tile_count;
-        }
-        __syncthreads();
+        own_off = pattern_offset<C::TE>(a, tile);
+    }
+    else if constexpr (MODE == kModeSingle) {
+        // finished once this tile's data loads are in flight
+        own_off = prefix_finish<C, ALIGNED>(a, tbase, pfx, sh);
+    }
+    if constexpr (kOwnOffset) {
+        if (t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = own_off + tile_count;
     }
     else if constexpr (MODE == kModeFused) {
         if (t < kWave) {
@@ -734,15 +734,6 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
                 sh.off = pre;
                 if (tile == ntiles - 1 && a.count_out) *a.count_out = pre + tile_count;
             }
-        }
-        __syncthreads();
-    }
-    else if constexpr (MODE == kModeSingle) {
-        // finished once this tile's data loads are in flight
-        const uint64_t pre = prefix_finish<C, ALIGNED>(a, tbase, pfx, sh);
-        if (t == 0) {
-            sh.off = pre;
-            if (tile == ntiles - 1 && a.count_out) *a.count_out = pre + tile_count;
         }
         __syncthreads();
     }
@@ -769,7 +760,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         if (t == 0) sh.off = (a.offsets ? a.offsets[chunk] : (uint64_t)a.offsets32[chunk]) + extra;
         __syncthreads();
     }
-    const uint64_t tile_off = MODE == kModeWriteout && a.bprefix ? bpre + uoff : sh.off;
+    const uint64_t tile_off = kOwnOffset ? own_off : MODE == kModeWriteout && a.bprefix ? bpre + uoff : sh.off;
     const uint64_t toff = tile_off + (a.out_base ? *a.out_base : 0ull);
 
     // ---- ranks, then staging (or direct stores) ----
