@@ -1046,7 +1046,7 @@ static_assert(sizeof(csx_segment) == sizeof(csx::Segment), "segment layouts agre
 
 struct BatchPlan {
     uint64_t max_tiles, status_words;
-    size_t prefix_off, tcounts_off, tseg_off, goff_off, status_off, total;
+    size_t prefix_off, tcounts_off, rec_off, goff_off, status_off, total;
 };
 
 // the workspace is sized for the smallest tile (128 threads x 2 vectors), so
@@ -1060,8 +1060,8 @@ static BatchPlan plan_batch(uint32_t nseg, uint64_t max_total, uint32_t w, uint3
     p.status_words = csx::scan_tiles(p.max_tiles);
     p.prefix_off = sizeof(csx::WsHeader);
     p.tcounts_off = align_up(p.prefix_off + 8ull * (nseg + 1));
-    p.tseg_off = align_up(p.tcounts_off + 4 * p.max_tiles);  // each tile's segment
-    p.goff_off = align_up(p.tseg_off + 4 * p.max_tiles);
+    p.rec_off = align_up(p.tcounts_off + 4 * p.max_tiles);  // each tile's record
+    p.goff_off = align_up(p.rec_off + sizeof(csx::TileRec) * p.max_tiles);
     p.status_off = align_up(p.goff_off + 8 * p.max_tiles);
     p.total = align_up(p.status_off + 8 * (p.status_words + 1));
     return p;
@@ -1094,7 +1094,7 @@ static csx_status batch_impl(const csx_segment* d_segments, uint32_t nseg, uint6
     CSX_HIP(timed_span(stream, [&] {
                 return csx::launch_batch(elem_size, bit_order, reinterpret_cast<const csx::Segment*>(d_segments), nseg, p.max_tiles, d_counts,
                                          reinterpret_cast<uint64_t*>(base + p.prefix_off), reinterpret_cast<uint32_t*>(base + p.tcounts_off),
-                                         reinterpret_cast<uint32_t*>(base + p.tseg_off), reinterpret_cast<uint64_t*>(base + p.goff_off),
+                                         reinterpret_cast<csx::TileRec*>(base + p.rec_off), reinterpret_cast<uint64_t*>(base + p.goff_off),
                                          reinterpret_cast<csx::WsHeader*>(base),
                                          reinterpret_cast<uint64_t*>(base + p.status_off), stream, cfg.vpt, count_blocks, block);
             }),

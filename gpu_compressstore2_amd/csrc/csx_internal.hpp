@@ -89,7 +89,7 @@ hipError_t launch_carry(uint64_t* dst, const uint64_t* base, const uint64_t* tot
 uint64_t batch_tile_elems(uint32_t w, uint32_t vpt, uint32_t block);
 uint64_t batch_max_tiles(uint32_t w, uint32_t nseg, uint64_t max_total, uint32_t vpt, uint32_t block);
 hipError_t launch_batch(uint32_t w, int order, const Segment* segs, uint32_t nseg, uint64_t max_tiles, uint64_t* counts, uint64_t* tile_prefix,
-                        uint32_t* tcounts, uint32_t* tseg, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s, uint32_t vpt = 8,
+                        uint32_t* tcounts, TileRec* rec, uint64_t* goff, WsHeader* hdr, uint64_t* status, hipStream_t s, uint32_t vpt = 8,
                         uint32_t count_blocks = 4096, uint32_t block = 256);
 // device generators (csx_gen.hip)
 hipError_t launch_gen_bernoulli(uint8_t* mask, uint64_t n, uint64_t thr, uint64_t seed, int order, hipStream_t s);

@@ -648,7 +648,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         // still publish its look-back status)
         if (tile_count == 0) {
             if (MODE == kModeTwoPass && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = prefix_total;
-            if (MODE == kModeGiven && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = a.given_off;
+            if (MODE == kModeGiven && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = a.given_off - a.given_base;
             if (MODE == kModePattern && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = pattern_offset<C::TE>(a, tile);
             if constexpr (MODE == kModeSingle) {
                 if (tile == ntiles - 1 && a.count_out) {
@@ -707,7 +707,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         own_off = prefix_total;
     }
     else if constexpr (MODE == kModeGiven) {
-        own_off = a.given_off;
+        own_off = a.given_off - a.given_base;
     }
     else if constexpr (MODE == kModePattern) {
         own_off = pattern_offset<C::TE>(a, tile);

@@ -39,6 +39,7 @@ inline uint32_t popb_log2(uint64_t n)
 inline uint64_t popb_blocks(uint64_t n) { return (n + (1ull << popb_log2(n)) - 1) >> popb_log2(n); }
 
 // kModeGiven: the tile's output offset is supplied in TileArgs::given_off
+// (less TileArgs::given_base)
 // kModePattern: no mask in memory; mask words and the tile's offset follow
 // from a periodic pattern (TileArgs::pat_*)
 // kModeSingle: the tile's offset is the popcount of the whole mask before
@@ -74,8 +75,11 @@ struct TileArgs {
     const uint32_t* tcounts;
     const uint64_t* gsums;
     uint32_t log2g;
-    // given offset (kModeGiven)
+    // given offset (kModeGiven): given_off - given_base, subtracted where the
+    // offset is used (at the stores), so that a caller's load of either value
+    // is waited for there rather than ahead of the tile's mask load
     uint64_t given_off;
+    uint64_t given_base;
     // periodic pattern (kModePattern): bit k of pat_bits (LSB first) is the
     // selection of element k for k < 64, a period of pat_len elements with
     // pat_pop selected
@@ -91,6 +95,19 @@ struct Segment {
     void* out;
     uint64_t n;
 };
+
+// One tile of a batched compaction, written by the count kernel: what the
+// write kernels need to address the tile's mask and data, so that those loads
+// wait for one record instead of the tile's segment index and then its
+// descriptor.  Tile indices are below 2^24 (one launch), so `first` fits.
+struct TileRec {
+    const void* in;       // the segment's input
+    const uint8_t* mask;  // the segment's mask
+    uint64_t n;           // the segment's length
+    uint32_t seg;         // the segment's index
+    uint32_t first;       // the segment's first tile (index over all tiles)
+};
+static_assert(sizeof(TileRec) == 32, "a record is one 32-byte store");
 
 // A predicate as one range test over the element type (csx_predicate.hip):
 // selected = (!(flags & kPredLo) || v >(=) lo) && (!(flags & kPredHi) || v <(=) hi),

@@ -81,6 +81,32 @@ def test_batch_every_tile_size_and_count_grid(O, w):
 
 
 @pytest.mark.parametrize("w", [1, 2, 4, 8])
+def test_batch_many_short_columns(O, w):
+    """Hundreds of columns shorter than a tile, a third of them empty, then a
+    long one: a wave load of the count kernel holds tiles of several columns
+    (up to 16 with 2-vector tiles) and its cursor walks them and the empty
+    ones between within one load; each tile's record is checked through the
+    write kernels' output."""
+    rng = np.random.default_rng(300 + w)
+    lengths = [0 if rng.random() < 0.33 else int(rng.integers(1, 300)) for _ in range(400)] + [3 * TILE[w] + 11]
+    for order in (oracle.MSB_FIRST, oracle.LSB_FIRST):
+        cols, ins, masks, outs = build(O, w, lengths, seed=60 + w + order, order=order)
+        table = csx.segment_table(ins, masks, outs, lengths)
+        total = sum(lengths)
+        ws = csx.alloc_workspace(csx.batch_workspace_size(len(lengths), total, w))
+        counts = torch.zeros(len(lengths), dtype=torch.int64, device="cuda")
+        for vpt, block in ((2, 128), (2, 256), (4, 512), (8, 512)):
+            for cb in (1, 3, 4096):
+                print(f"batch short columns w={w} order={order} vpt={vpt} block={block} count_blocks={cb}", flush=True)
+                for o in outs:
+                    o.fill_(-1)
+                csx.compact_batch_config(table, w, total, csx.Config(csx.PIPELINE_THREE_PHASE, 0, vpt, cb, block), counts=counts, workspace=ws,
+                                         bit_order=order)
+                torch.cuda.synchronize()
+                check(cols, outs, counts, w)
+
+
+@pytest.mark.parametrize("w", [1, 2, 4, 8])
 def test_batch_unaligned_segments(O, w):
     """Columns sliced at offsets that break 16-byte alignment: input and
     output off by whole elements, masks off by 1..15 bytes, every third
