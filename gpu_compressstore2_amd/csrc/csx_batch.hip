@@ -405,8 +405,11 @@ __global__ __launch_bounds__(256) void k_batch_sparse(const Segment* segs, uint3
     for (uint64_t base = wave; base < total; base += kWave * waves) {
         const uint64_t gl = base + lane * waves;
         const uint32_t tc = gl < total ? tcounts[gl] : 0u;
-        const uint32_t sl = gl < total ? rec[gl].seg : 0u;  // the lane's tile's segment
-        uint64_t todo = __ballot(tc >= 1 && tc <= THR);
+        const bool sparse = tc >= 1 && tc <= THR;
+        // the lane's tile's segment, read only for a sparse tile: a batch
+        // with none reads the counts alone
+        const uint32_t sl = sparse ? rec[gl].seg : 0u;
+        uint64_t todo = __ballot(sparse);
         while (todo) {
             const uint32_t idx = (uint32_t)__builtin_ctzll(todo);
             const uint64_t g = base + (uint64_t)idx * waves;
