@@ -210,6 +210,33 @@ def test_batch_bound_too_small_sets_error(O):
     assert csx.workspace_error(ws) == 2
 
 
+@pytest.mark.parametrize("w", [1, 4, 8])
+def test_batch_loose_bound_dirty_workspace(O, w):
+    """A bound far above the columns' total and a workspace full of 0xFF: the
+    surplus tiles of the bound find records and counts nobody wrote and must
+    leave before using them.  A second, smaller batch in the same workspace
+    then meets the first batch's stale records past its own total."""
+    rng = np.random.default_rng(90 + w)
+    T = TILE[w]
+    lengths = [int(x) for x in rng.integers(0, 3 * T, 30)] + [5 * T + 3]
+    small = lengths[::3]
+    bound = 4 * sum(lengths)
+    ws = csx.alloc_workspace(csx.batch_workspace_size(len(lengths), bound, w))
+    ws.fill_(0xFF)
+    for lens, seed in ((lengths, 21), (small, 22)):
+        cols, ins, masks, outs = build(O, w, lens, seed=seed + w)
+        table = csx.segment_table(ins, masks, outs, lens)
+        counts = torch.zeros(len(lens), dtype=torch.int64, device="cuda")
+        for vpt, block in ((2, 128), (8, 512)):
+            print(f"batch loose bound w={w} nseg={len(lens)} vpt={vpt} block={block}", flush=True)
+            for o in outs:
+                o.fill_(-1)
+            csx.compact_batch_config(table, w, bound, csx.Config(csx.PIPELINE_THREE_PHASE, 0, vpt, 64, block), counts=counts, workspace=ws)
+            torch.cuda.synchronize()
+            check(cols, outs, counts, w)
+            assert csx.workspace_error(ws) == 0
+
+
 def test_batch_graph_capture_replays(O):
     lengths = [5000, 70000, 123, 9000, 0, 40000]
     cols, ins, masks, outs = build(O, 4, lengths, seed=8)
