@@ -768,14 +768,21 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
         const uint64_t te = pipeline == CSX_PIPELINE_SPARSE ? csx::kSparseStepElems : wte;
         const uint32_t log2unit = std::min(log2u(chunk_len), log2u(te));
         const uint32_t log2b = csx::popb_log2(n);
+        // a column of few blocks skips the block prefixes: each writeout
+        // wave sums the totals before its block, and the last tile writes
+        // the count (the sparse writeout keeps the prefixes)
+        const bool few_blocks = pipeline != CSX_PIPELINE_SPARSE && csx::popb_blocks(n) <= csx::kMidBlocks && n < (1ull << 32);
         CSX_HIP(csx::launch_popc_blocks(d_mask, n, log2b, log2unit, uoffs, btotals, bit_order, stream, hdr), "csx_compact popc");
-        CSX_HIP(csx::launch_block_prefix(btotals, csx::popb_blocks(n), bprefix, count_dst, n >= (1ull << 32), stream), "csx_compact block prefixes");
+        if (!few_blocks)
+            CSX_HIP(csx::launch_block_prefix(btotals, csx::popb_blocks(n), bprefix, count_dst, n >= (1ull << 32), stream), "csx_compact block prefixes");
         csx::TileArgs a{};
         a.in = in;
         a.mask = d_mask;
         a.out = out;
         a.n = n;
-        a.bprefix = bprefix;
+        a.bprefix = few_blocks ? nullptr : bprefix;
+        a.btotals = few_blocks ? btotals : nullptr;
+        a.count_out = few_blocks ? count_dst : nullptr;
         a.uoffs = uoffs;
         a.log2b = log2b;
         a.log2c = log2unit;

@@ -37,6 +37,9 @@ inline uint32_t popb_log2(uint64_t n)
     return l;
 }
 inline uint64_t popb_blocks(uint64_t n) { return (n + (1ull << popb_log2(n)) - 1) >> popb_log2(n); }
+// columns of up to this many blocks skip the block-prefix kernel: each
+// writeout wave sums the totals before its block, two per lane
+constexpr uint32_t kMidBlocks = 128;
 
 // kModeGiven: the tile's output offset is supplied in TileArgs::given_off
 // (less TileArgs::given_base)
@@ -67,6 +70,11 @@ struct TileArgs {
     const uint64_t* bprefix;
     const uint32_t* uoffs;
     uint32_t log2b;
+    // instead of bprefix, for columns of at most kMidBlocks blocks (and a
+    // total below 2^32): the blocks' selected counts, summed by every wave
+    // of a tile for the blocks before its own, and the total count written
+    // by the last tile to count_out (no block-prefix kernel)
+    const uint32_t* btotals;
     // fused
     WsHeader* hdr;
     uint64_t* status;
