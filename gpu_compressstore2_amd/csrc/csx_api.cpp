@@ -767,11 +767,18 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
         auto bprefix = reinterpret_cast<uint64_t*>(base + p.bpre_off);
         const uint64_t te = pipeline == CSX_PIPELINE_SPARSE ? csx::kSparseStepElems : wte;
         const uint32_t log2unit = std::min(log2u(chunk_len), log2u(te));
-        const uint32_t log2b = csx::popb_log2(n);
-        // a column of few blocks skips the block prefixes: each writeout
+        // A column of few blocks skips the block prefixes: each writeout
         // wave sums the totals before its block, and the last tile writes
-        // the count (the sparse writeout keeps the prefixes)
-        const bool few_blocks = pipeline != CSX_PIPELINE_SPARSE && csx::popb_blocks(n) <= csx::kMidBlocks && n < (1ull << 32);
+        // the count (the sparse writeout keeps the prefixes).  Blocks grow
+        // up to 2^19 elements (one round of a 1024-thread popcount
+        // workgroup) to stay within kMidBlocks: up to 2^26 elements.
+        uint32_t log2b = csx::popb_log2(n);
+        bool few_blocks = false;
+        if (pipeline != CSX_PIPELINE_SPARSE && n < (1ull << 32)) {
+            while (log2b < csx::kPopbMinLog2 + 2 && ((n + (1ull << log2b) - 1) >> log2b) > csx::kMidBlocks) log2b++;
+            few_blocks = ((n + (1ull << log2b) - 1) >> log2b) <= csx::kMidBlocks;
+            if (!few_blocks) log2b = csx::popb_log2(n);
+        }
         CSX_HIP(csx::launch_popc_blocks(d_mask, n, log2b, log2unit, uoffs, btotals, bit_order, stream, hdr), "csx_compact popc");
         if (!few_blocks)
             CSX_HIP(csx::launch_block_prefix(btotals, csx::popb_blocks(n), bprefix, count_dst, n >= (1ull << 32), stream), "csx_compact block prefixes");
@@ -791,7 +798,8 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
             CSX_HIP(timed(stream, [&] { return dispatch_sparse(elem_size, bit_order, mask_aligned, a, stream); }), "csx_compact sparse writeout");
         }
         else {
-            CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, csx::kModeWriteout, true, bit_order, aligned, a, stream, (int)vpt, (int)block); }),
+            const int mode = few_blocks ? csx::kModeFewBlocks : csx::kModeWriteout;
+            CSX_HIP(timed(stream, [&] { return dispatch_tile(elem_size, mode, true, bit_order, aligned, a, stream, (int)vpt, (int)block); }),
                     "csx_compact writeout");
         }
     }

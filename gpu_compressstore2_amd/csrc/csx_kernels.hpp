@@ -526,17 +526,16 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         bpre = *(a.bprefix ? a.bprefix + (tbase >> a.log2b) : kZeros);
         uoff = *(a.uoffs ? a.uoffs + (tbase >> a.log2c) : reinterpret_cast<const uint32_t*>(kZeros));
     }
-    // Without block prefixes (a column of at most kMidBlocks blocks): the
-    // totals of the blocks before this tile's, two per lane, requested ahead
-    // of the mask so that its wait covers them; each wave sums them where the
-    // offset is needed.
+    // Without block prefixes (kModeFewBlocks, a column of at most kMidBlocks
+    // blocks): the totals of the blocks before this tile's, two per lane,
+    // requested ahead of the mask so that its wait covers them; each wave
+    // sums them where the offset is needed.
     uint32_t btot0 = 0, btot1 = 0;  // added only at the offset: an add here would wait for them
-    if constexpr (MODE == kModeWriteout) {
-        if (a.btotals) {  // uniform
-            const uint32_t blk = (uint32_t)(tbase >> a.log2b), l = lane_id();
-            if (l < blk) btot0 = a.btotals[l];
-            if (l + kWave < blk) btot1 = a.btotals[l + kWave];
-        }
+    if constexpr (MODE == kModeFewBlocks) {
+        uoff = *(a.uoffs + (tbase >> a.log2c));
+        const uint32_t blk = (uint32_t)(tbase >> a.log2b), l = lane_id();
+        if (l < blk) btot0 = a.btotals[l];
+        if (l + kWave < blk) btot1 = a.btotals[l + kWave];
     }
     // this wave's sum of the totals (all 64 lanes active)
     auto blocks_before = [&] { return (uint64_t)wave_sum_u32(btot0 + btot1); };
@@ -664,8 +663,8 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
             if (MODE == kModeTwoPass && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = prefix_total;
             if (MODE == kModeGiven && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = a.given_off - a.given_base;
             if (MODE == kModePattern && t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = pattern_offset<C::TE>(a, tile);
-            if constexpr (MODE == kModeWriteout) {
-                if (a.btotals && tile == ntiles - 1 && a.count_out) {
+            if constexpr (MODE == kModeFewBlocks) {
+                if (tile == ntiles - 1 && a.count_out) {
                     const uint64_t total = blocks_before() + uoff;
                     if (t == 0) *a.count_out = total;
                 }
@@ -757,13 +756,13 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         }
         __syncthreads();
     }
-    else if (a.bprefix) {
-        // every thread holds bpre and uoff: no barrier here
-    }
-    else if (MODE == kModeWriteout && a.btotals) {
+    else if constexpr (MODE == kModeFewBlocks) {
         // every wave sums the totals before its block: no barrier either
         bpre = blocks_before();
         if (t == 0 && tile == ntiles - 1 && a.count_out) *a.count_out = bpre + uoff + tile_count;
+    }
+    else if (a.bprefix) {
+        // every thread holds bpre and uoff: no barrier here
     }
     else {
         const uint64_t chunk = tbase >> a.log2c;
@@ -785,7 +784,7 @@ __device__ __forceinline__ void tile_body(const TileArgs& a, TileShared<TileCfg<
         if (t == 0) sh.off = (a.offsets ? a.offsets[chunk] : (uint64_t)a.offsets32[chunk]) + extra;
         __syncthreads();
     }
-    const uint64_t tile_off = kOwnOffset ? own_off : MODE == kModeWriteout && (a.bprefix || a.btotals) ? bpre + uoff : sh.off;
+    const uint64_t tile_off = kOwnOffset ? own_off : MODE == kModeFewBlocks || (MODE == kModeWriteout && a.bprefix) ? bpre + uoff : sh.off;
     const uint64_t toff = tile_off + (a.out_base ? *a.out_base : 0ull);
 
     // ---- ranks, then staging (or direct stores) ----
@@ -846,7 +845,7 @@ __global__ __launch_bounds__(BLOCK) void k_tile(TileArgs a)
         __syncthreads();
         tile = sh.tile;
     }
-    else if constexpr (MODE == kModeWriteout || MODE == kModePattern || MODE == kModeSingle) {
+    else if constexpr (MODE == kModeWriteout || MODE == kModeFewBlocks || MODE == kModePattern || MODE == kModeSingle) {
         // the grid is exactly one block per tile
         tile = xcd_runs(blockIdx.x, gridDim.x);
         // the single launch clears the workspace error word (nothing of it sets

@@ -9,20 +9,35 @@
 
 namespace csx {
 
-template <int W, int ORDER, bool ALIGNED, int VPT, int BLOCK> static void launch_writeout_shape(const TileArgs& a, hipStream_t s)
+template <int W, int ORDER, bool ALIGNED, int MODE, int VPT, int BLOCK> static void launch_writeout_shape(const TileArgs& a, hipStream_t s)
 {
     using C = TileCfg<W, VPT, BLOCK>;
     const uint64_t tiles = (a.n + C::TE - 1) / C::TE;
-    launch_k(k_tile<W, ORDER, ALIGNED, kModeWriteout, true, VPT, BLOCK>, dim3((unsigned)tiles), dim3(BLOCK), s, a);
+    launch_k(k_tile<W, ORDER, ALIGNED, MODE, true, VPT, BLOCK>, dim3((unsigned)tiles), dim3(BLOCK), s, a);
 }
 
-// the staged writeout's tile: `block` threads (128, 256 or 512) x VPT vectors
-template <int W, int ORDER, bool ALIGNED, int VPT> static void launch_writeout_vpt(const TileArgs& a, int block, hipStream_t s)
+// the staged writeout's tile (kModeWriteout or kModeFewBlocks): `block`
+// threads (128, 256 or 512) x VPT vectors
+template <int W, int ORDER, bool ALIGNED, int MODE, int VPT> static void launch_writeout_vpt(const TileArgs& a, int block, hipStream_t s)
 {
     switch (block) {
-        case 128: launch_writeout_shape<W, ORDER, ALIGNED, VPT, 128>(a, s); break;
-        case 512: launch_writeout_shape<W, ORDER, ALIGNED, VPT, 512>(a, s); break;
-        default: launch_writeout_shape<W, ORDER, ALIGNED, VPT, 256>(a, s); break;
+        case 128: launch_writeout_shape<W, ORDER, ALIGNED, MODE, VPT, 128>(a, s); break;
+        case 512: launch_writeout_shape<W, ORDER, ALIGNED, MODE, VPT, 512>(a, s); break;
+        default: launch_writeout_shape<W, ORDER, ALIGNED, MODE, VPT, 256>(a, s); break;
+    }
+}
+
+template <int W, int ORDER, bool ALIGNED, int MODE> static void launch_writeout_mode(const TileArgs& a, int vpt, int block, hipStream_t s)
+{
+    // three-phase writeout: a tile's offset comes from the chunk offsets (or
+    // the block totals), so any tile size works with the same workspace
+    switch (vpt) {
+        case 2: launch_writeout_vpt<W, ORDER, ALIGNED, MODE, 2>(a, block, s); break;
+        case 4: launch_writeout_vpt<W, ORDER, ALIGNED, MODE, 4>(a, block, s); break;
+        case 16:
+            if constexpr (W >= 2) launch_writeout_vpt<W, ORDER, ALIGNED, MODE, 16>(a, block, s);
+            break;
+        default: launch_writeout_vpt<W, ORDER, ALIGNED, MODE, 8>(a, block, s); break;
     }
 }
 
@@ -80,17 +95,11 @@ static void launch_tile_impl(int mode, bool staged, int vpt, int block, const Ti
             }
         }
     }
+    else if (mode == kModeFewBlocks) {
+        launch_writeout_mode<W, ORDER, ALIGNED, kModeFewBlocks>(a, vpt, block, s);
+    }
     else if (staged) {
-        // three-phase writeout: a tile's offset comes from the chunk offsets,
-        // so any tile size works with the same workspace
-        switch (vpt) {
-            case 2: launch_writeout_vpt<W, ORDER, ALIGNED, 2>(a, block, s); break;
-            case 4: launch_writeout_vpt<W, ORDER, ALIGNED, 4>(a, block, s); break;
-            case 16:
-                if constexpr (W >= 2) launch_writeout_vpt<W, ORDER, ALIGNED, 16>(a, block, s);
-                break;
-            default: launch_writeout_vpt<W, ORDER, ALIGNED, 8>(a, block, s); break;
-        }
+        launch_writeout_mode<W, ORDER, ALIGNED, kModeWriteout>(a, vpt, block, s);
     }
     else {
         launch_k(k_tile<W, ORDER, ALIGNED, kModeWriteout, false>, dim3(grid), dim3(256), s, a);
@@ -103,7 +112,8 @@ template <int W> hipError_t launch_tile(int mode, bool staged, int order, bool a
     if (mode == kModeSingle && vpt == 16) return hipErrorInvalidValue;
     // only the staged writeout's tile takes another workgroup size than 256
     if (block != 128 && block != 256 && block != 512) return hipErrorInvalidValue;
-    if (block != 256 && !(mode == kModeWriteout && staged)) return hipErrorInvalidValue;
+    if (block != 256 && !((mode == kModeWriteout || mode == kModeFewBlocks) && staged)) return hipErrorInvalidValue;
+    if (mode == kModeFewBlocks && (!staged || !a.btotals || !a.uoffs)) return hipErrorInvalidValue;
     const uint64_t tiles = (a.n + kTileElems<W> - 1) / kTileElems<W>;
     if (tiles == 0) return hipSuccess;
     const unsigned grid = (unsigned)tiles;

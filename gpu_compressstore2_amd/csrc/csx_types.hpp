@@ -47,7 +47,9 @@ constexpr uint32_t kMidBlocks = 128;
 // from a periodic pattern (TileArgs::pat_*)
 // kModeSingle: the tile's offset is the popcount of the whole mask before
 // it, counted by the tile itself (one launch, no other kernel's results)
-enum TileMode { kModeWriteout = 0, kModeFused = 1, kModeTwoPass = 2, kModeGiven = 3, kModePattern = 4, kModeSingle = 5 };
+// kModeFewBlocks: the three-phase writeout of a column of at most kMidBlocks
+// popcount blocks, with no block prefixes (TileArgs::btotals)
+enum TileMode { kModeWriteout = 0, kModeFused = 1, kModeTwoPass = 2, kModeGiven = 3, kModePattern = 4, kModeSingle = 5, kModeFewBlocks = 6 };
 
 // Arguments of one tile-engine launch.
 struct TileArgs {
@@ -70,10 +72,10 @@ struct TileArgs {
     const uint64_t* bprefix;
     const uint32_t* uoffs;
     uint32_t log2b;
-    // instead of bprefix, for columns of at most kMidBlocks blocks (and a
-    // total below 2^32): the blocks' selected counts, summed by every wave
-    // of a tile for the blocks before its own, and the total count written
-    // by the last tile to count_out (no block-prefix kernel)
+    // kModeFewBlocks, instead of bprefix, for columns of at most kMidBlocks
+    // blocks (and a total below 2^32): the blocks' selected counts, summed by
+    // every wave of a tile for the blocks before its own, and the total count
+    // written by the last tile to count_out (no block-prefix kernel)
     const uint32_t* btotals;
     // fused
     WsHeader* hdr;
