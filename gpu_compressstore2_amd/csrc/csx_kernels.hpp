@@ -212,7 +212,10 @@ __global__ __launch_bounds__(kScanBlock) void k_scan(const uint32_t* counts, Out
 // element.  The block's selected count goes to btotals[b].
 //
 // K2 (k_block_prefix).  One workgroup turns the block counts into exclusive
-// block prefixes (u64) and writes the total selected count.
+// block prefixes (u64) and writes the total selected count.  A column of at
+// most kMidBlocks blocks skips it: its writeout (kModeFewBlocks) sums the
+// block counts before each tile's block in every wave, and its last tile
+// writes the total.
 //
 // No workgroup waits for another, so there is no look-back state to
 // initialise; a writeout tile's output offset is bprefix[its block] +
@@ -342,8 +345,10 @@ __global__ __launch_bounds__(THREADS) void k_popc_blocks(const uint8_t* __restri
 //   * data: each thread owns VPT 16-byte vectors, striped across the block
 //     (vector j*256 + t) so every load instruction of a wave reads 1 KiB
 //     contiguous; a vector whose mask bits are all zero is not loaded;
-//   * the tile's output offset comes from the chunk offsets (three-phase
-//     writeout), a decoupled look-back over tile counts (fused), the tile
+//   * the tile's output offset comes from the chunk offsets or the block
+//     prefixes and unit offsets (three-phase writeout), the block counts
+//     before its block and its unit offset (few blocks), a decoupled
+//     look-back over tile counts (fused), the tile
 //     and group counts (two-pass), the caller (given) or, with no mask in
 //     memory, a periodic pattern whose mask words the tile generates;
 //   * STAGED: selected elements are written to LDS at their rank, shifted so
