@@ -768,13 +768,13 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
         const uint64_t te = pipeline == CSX_PIPELINE_SPARSE ? csx::kSparseStepElems : wte;
         const uint32_t log2unit = std::min(log2u(chunk_len), log2u(te));
         // A column of few blocks skips the block prefixes: each writeout
-        // wave sums the totals before its block, and the last tile writes
-        // the count (the sparse writeout keeps the prefixes).  Blocks grow
-        // up to 2^19 elements (one round of a 1024-thread popcount
-        // workgroup) to stay within kMidBlocks: up to 2^26 elements.
+        // wave (tile or sparse step) sums the totals before its block, and
+        // the last tile or step writes the count.  Blocks grow up to 2^19
+        // elements (one round of a 1024-thread popcount workgroup) to stay
+        // within kMidBlocks: up to 2^26 elements.
         uint32_t log2b = csx::popb_log2(n);
         bool few_blocks = false;
-        if (pipeline != CSX_PIPELINE_SPARSE && n < (1ull << 32)) {
+        if (n < (1ull << 32)) {
             while (log2b < csx::kPopbMinLog2 + 2 && ((n + (1ull << log2b) - 1) >> log2b) > csx::kMidBlocks) log2b++;
             few_blocks = ((n + (1ull << log2b) - 1) >> log2b) <= csx::kMidBlocks;
             if (!few_blocks) log2b = csx::popb_log2(n);

@@ -987,7 +987,10 @@ __global__ __launch_bounds__(256) void k_thread_per_chunk(TileArgs a)
 // per-element gathers lose to whole-vector loads.
 constexpr uint64_t kSparseStep = kSparseStepElems;
 
-template <int W, int ORDER, bool ALIGNED>
+// FEW: a column of at most kMidBlocks blocks without block prefixes (as the
+// tile engine's kModeFewBlocks): each step's wave sums the block counts
+// before its block, and the wave of the last step writes the total count.
+template <int W, int ORDER, bool ALIGNED, bool FEW = false>
 __global__ __launch_bounds__(256) void k_sparse(TileArgs a)
 {
     const uint8_t* __restrict__ in = a.in;
@@ -1004,8 +1007,16 @@ __global__ __launch_bounds__(256) void k_sparse(TileArgs a)
         // two-level offsets: the step's block prefix and unit offset,
         // requested ahead of the mask with no branch around them (an absent
         // array reads kZeros), so that no wait for them precedes the mask load
-        const uint64_t bpre = *(a.bprefix ? a.bprefix + (e0 >> a.log2b) : kZeros);
+        const uint64_t bpre = FEW ? 0ull : *(a.bprefix ? a.bprefix + (e0 >> a.log2b) : kZeros);
         const uint32_t uoff = *(a.uoffs ? a.uoffs + (e0 >> a.log2c) : reinterpret_cast<const uint32_t*>(kZeros));
+        // FEW: the counts of the blocks before the step's, two per lane, also
+        // ahead of the mask (added only where the offset is needed)
+        uint32_t btot0 = 0, btot1 = 0;
+        if constexpr (FEW) {
+            const uint32_t blk = (uint32_t)(e0 >> a.log2b);
+            if (lane < blk) btot0 = a.btotals[lane];
+            if (lane + kWave < blk) btot1 = a.btotals[lane + kWave];
+        }
         uint32_t w[4] = {0, 0, 0, 0};
         if (el < n) {
             const u32x4 x = load_mask16<ALIGNED>(a.mask, el / 8, nbytes);
@@ -1028,21 +1039,31 @@ __global__ __launch_bounds__(256) void k_sparse(TileArgs a)
         const uint64_t chunk = e0 >> a.log2c;
         const uint64_t cstart = chunk << a.log2c;
         uint64_t before = 0, step_off = 0;
-        if (a.bprefix) {
-            step_off = bpre + uoff;
+        if constexpr (FEW) {
+            step_off = (uint64_t)wave_sum_u32(btot0 + btot1) + uoff;
         }
-        else if (cstart < e0) {
-            // whole 16-byte vectors below e0 < n: no tail bits among them
-            uint32_t c = 0;
-            for (uint64_t v = cstart / 128 + lane; v < e0 / 128; v += kWave) {
-                const u32x4 x = load_mask16<ALIGNED>(a.mask, v * 16, nbytes);
-                c += popc32(x.x) + popc32(x.y) + popc32(x.z) + popc32(x.w);
+        else {
+            if (a.bprefix) {
+                step_off = bpre + uoff;
             }
-            before = wave_sum((uint64_t)c);
+            else if (cstart < e0) {
+                // whole 16-byte vectors below e0 < n: no tail bits among them
+                uint32_t c = 0;
+                for (uint64_t v = cstart / 128 + lane; v < e0 / 128; v += kWave) {
+                    const u32x4 x = load_mask16<ALIGNED>(a.mask, v * 16, nbytes);
+                    c += popc32(x.x) + popc32(x.y) + popc32(x.z) + popc32(x.w);
+                }
+                before = wave_sum((uint64_t)c);
+            }
+            if (!a.bprefix) step_off = (a.offsets ? a.offsets[chunk] : (uint64_t)a.offsets32[chunk]) + before;
         }
-        if (!a.bprefix) step_off = (a.offsets ? a.offsets[chunk] : (uint64_t)a.offsets32[chunk]) + before;
         const uint32_t cnt = popc32(w[0]) + popc32(w[1]) + popc32(w[2]) + popc32(w[3]);
-        uint64_t r = base_out + step_off + wave_inclusive_sum_u32(cnt) - cnt;
+        const uint32_t inc = wave_inclusive_sum_u32(cnt);
+        if constexpr (FEW) {
+            // the last step's lane 63 holds the step's count: the column's total follows
+            if (step == steps - 1 && lane == kWave - 1 && a.count_out) *a.count_out = step_off + inc;
+        }
+        uint64_t r = base_out + step_off + inc - cnt;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             uint32_t bits = w[k];

@@ -167,24 +167,34 @@ template <int W> hipError_t launch_thread_per_chunk(int order, const TileArgs& a
     return hipGetLastError();
 }
 
+template <int W, bool FEW> static void launch_sparse_few(int order, bool mask_aligned, const TileArgs& a, unsigned g, hipStream_t s)
+{
+    if (order == kBitMsb) {
+        if (mask_aligned)
+            launch_k(k_sparse<W, kBitMsb, true, FEW>, dim3(g), dim3(256), s, a);
+        else
+            launch_k(k_sparse<W, kBitMsb, false, FEW>, dim3(g), dim3(256), s, a);
+    }
+    else {
+        if (mask_aligned)
+            launch_k(k_sparse<W, kBitLsb, true, FEW>, dim3(g), dim3(256), s, a);
+        else
+            launch_k(k_sparse<W, kBitLsb, false, FEW>, dim3(g), dim3(256), s, a);
+    }
+}
+
+// with a.btotals (a column of at most kMidBlocks blocks, no block prefixes)
+// the waves sum the block counts themselves and the last step writes the count
 template <int W> hipError_t launch_sparse(int order, bool mask_aligned, const TileArgs& a, hipStream_t s)
 {
     const uint64_t steps = (a.n + kSparseStep - 1) / kSparseStep;
     uint64_t g = (steps + 3) / 4;  // one step per wave
     if (g < 1) g = 1;
     if (g > (1u << 22)) g = 1u << 22;  // beyond 2^37 elements the waves loop
-    if (order == kBitMsb) {
-        if (mask_aligned)
-            launch_k(k_sparse<W, kBitMsb, true>, dim3((unsigned)g), dim3(256), s, a);
-        else
-            launch_k(k_sparse<W, kBitMsb, false>, dim3((unsigned)g), dim3(256), s, a);
-    }
-    else {
-        if (mask_aligned)
-            launch_k(k_sparse<W, kBitLsb, true>, dim3((unsigned)g), dim3(256), s, a);
-        else
-            launch_k(k_sparse<W, kBitLsb, false>, dim3((unsigned)g), dim3(256), s, a);
-    }
+    if (a.btotals)
+        launch_sparse_few<W, true>(order, mask_aligned, a, (unsigned)g, s);
+    else
+        launch_sparse_few<W, false>(order, mask_aligned, a, (unsigned)g, s);
     return hipGetLastError();
 }
 

@@ -305,13 +305,14 @@ def test_three_phase_workgroup_sizes(O, w, block):
 
 
 @pytest.mark.parametrize("w", [1, 4, 8])
-def test_three_phase_few_blocks_edges(O, w):
+def test_few_blocks_edges(O, w):
     """Columns of at most 128 popcount blocks skip the block prefixes: every
-    writeout wave sums the totals before its block and the last tile writes
-    the count.  Sizes where the blocks grow to 2^18 and 2^19 elements, the
-    largest such column (2^26: 128 blocks) and one element more (the prefix
-    kernel again); the last 300001 elements unselected, so the last tiles
-    are empty and the count comes from an empty tile."""
+    writeout wave (staged tile or sparse step) sums the totals before its
+    block and the last tile or step writes the count.  Sizes where the blocks
+    grow to 2^18 and 2^19 elements, the largest such column (2^26: 128
+    blocks) and one element more (the prefix kernel again); the last 300001
+    elements unselected, so the last tiles are empty and the count comes
+    from an empty tile."""
     for n in ((1 << 24) + 12345, (1 << 25) + 99, 1 << 26, (1 << 26) + 1):
         data = O.fill_values(n, NP[w], seed=n % 1000 + w)
         d_in = dev(data)
@@ -321,10 +322,11 @@ def test_three_phase_few_blocks_edges(O, w):
             mask = dirty_tail(mask, n, order)
             expect, ke = O.compact(data, mask, n, order)
             d_mask, d_exp = dev(mask), dev(expect)
-            for vpt, block in ((4, 256), (8, 512)):
-                print(f"three-phase few blocks w={w} n={n} order={order} vpt={vpt} block={block}", flush=True)
+            for pipeline, chunk, vpt, block in ((csx.PIPELINE_THREE_PHASE, 1024, 4, 256), (csx.PIPELINE_THREE_PHASE, 1024, 8, 512),
+                                                (csx.PIPELINE_SPARSE, 4096, 8, 256)):
+                print(f"few blocks w={w} n={n} order={order} pipeline={pipeline} vpt={vpt} block={block}", flush=True)
                 d_out = torch.full(((n + 16) * w,), 0xC3, dtype=torch.uint8, device="cuda").view(d_in.dtype)
-                _, k = csx.compact_config(d_in, d_mask, csx.Config(csx.PIPELINE_THREE_PHASE, 1024, vpt, 0, block), n=n, bit_order=order, out=d_out)
+                _, k = csx.compact_config(d_in, d_mask, csx.Config(pipeline, chunk, vpt, 0, block), n=n, bit_order=order, out=d_out)
                 assert k == ke and torch.equal(d_out[:k], d_exp)
                 assert bool((d_out[k:].view(torch.uint8) == 0xC3).all())
 
