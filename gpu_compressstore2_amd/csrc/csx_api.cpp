@@ -231,7 +231,9 @@ hipError_t dispatch_sparse(uint32_t w, int order, bool mask_aligned, const csx::
     }
 }
 
-bool launch_fits(uint64_t tiles) { return tiles * 256ull < (1ull << 32); }
+// a grid of `tiles` workgroups of `threads` holds fewer than 2^32 work-items
+// (the dispatch's 32-bit grid size)
+bool launch_fits(uint64_t tiles, uint32_t threads = 256) { return tiles < (1ull << 32) && tiles * threads < (1ull << 32); }
 
 bool known_pipeline(int p)
 {
@@ -584,7 +586,7 @@ csx_status csx_writeout_ex(const csx_writeout_args* wa, hipStream_t stream)
     // the staged tile: block threads x 8 vectors
     const uint64_t te = (uint64_t)block * 8 * (16 / elem_size);
     const uint64_t tiles = (n + te - 1) / te;
-    if (!launch_fits(tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)tiles);
+    if (!launch_fits(tiles, block)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)tiles);
     const bool aligned = (((uintptr_t)wa->in | (uintptr_t)wa->mask) & 15) == 0;
     CSX_HIP(timed(stream,
                   [&] { return dispatch_tile(elem_size, csx::kModeWriteout, variant == CSX_WRITEOUT_STAGED, bit_order, aligned, a, stream, 8, (int)block); }),
@@ -688,7 +690,7 @@ static csx_status compact_impl(const void* d_in, const uint8_t* d_mask, void* d_
     // the writeout's grid is one workgroup per tile of the chosen size
     const uint64_t wte = (uint64_t)block * vpt * (16 / elem_size);
     const uint64_t wtiles = (n + wte - 1) / wte;
-    if (!launch_fits(p.fused_tiles) || !launch_fits(wtiles))
+    if (!launch_fits(p.fused_tiles) || !launch_fits(wtiles, block))
         return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)std::max(p.fused_tiles, wtiles));
 
     auto base = static_cast<uint8_t*>(d_ws);
@@ -1103,7 +1105,7 @@ static csx_status batch_impl(const csx_segment* d_segments, uint32_t nseg, uint6
     const uint32_t block = block_of(cfg.block_size);
     const BatchPlan p = plan_batch(nseg, max_total_elements, elem_size, cfg.vpt, block);
     if ((s = check_ws(d_ws, ws_bytes, p.total))) return s;
-    if (!launch_fits(p.max_tiles)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)p.max_tiles);
+    if (!launch_fits(p.max_tiles, block)) return fail(CSX_ERR_UNSUPPORTED, "%llu tiles exceed one launch", (unsigned long long)p.max_tiles);
     auto base = static_cast<uint8_t*>(d_ws);
     const uint32_t count_blocks = cfg.count_blocks ? cfg.count_blocks : 4096;
     CSX_HIP(timed_span(stream, [&] {
@@ -1371,7 +1373,7 @@ csx_status csx_compact_predicate(const void* d_in, uint64_t n, const csx_predica
         uint32_t vpt = 8, block = 256;
         const csx_config tc = lookup_config(w, n, -1.0);
         const uint64_t tte = (uint64_t)block_of(tc.block_size) * tc.vpt * (16 / w);
-        if (tc.pipeline == CSX_PIPELINE_THREE_PHASE && launch_fits((n + tte - 1) / tte)) {
+        if (tc.pipeline == CSX_PIPELINE_THREE_PHASE && launch_fits((n + tte - 1) / tte, block_of(tc.block_size))) {
             vpt = tc.vpt;
             block = block_of(tc.block_size);
         }

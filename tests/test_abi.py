@@ -127,6 +127,22 @@ def test_single_launch_cap_rejected_before_device_work():
     assert b"single-launch" in L.csx_last_error()
 
 
+def test_grid_limit_counts_the_workgroup_size():
+    """A writeout grid must hold fewer than 2^32 work-items.  2^35 u32
+    elements in 2-vector tiles of 512 threads (4096 elements) make 2^23
+    tiles, 2^32 work-items: refused before any launch, although 2^23
+    workgroups of 256 threads would fit (fake addresses, never
+    dereferenced)."""
+    L = csx.lib()
+    n, w = 1 << 35, 4
+    V = ctypes.c_void_p
+    data, mask, out, ws = 1 << 44, 1 << 46, 1 << 47, 1 << 43
+    need = csx.workspace_size(n, w, 1024)
+    cfg = csx.Config(csx.PIPELINE_THREE_PHASE, 1024, 2, 0, 512)
+    st = L.csx_compact_config(V(data), V(mask), V(out), n, w, 0, ctypes.byref(cfg), None, None, V(ws), need, None)
+    assert st == 4 and b"exceed one launch" in L.csx_last_error()
+
+
 def test_in_place_calls_rejected_before_device_work():
     """Compaction is not in place: an output range that shares a byte with
     the input or the mask is refused with CSX_ERR_INVALID before any launch
