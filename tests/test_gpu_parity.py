@@ -331,6 +331,32 @@ def test_few_blocks_edges(O, w):
                 assert bool((d_out[k:].view(torch.uint8) == 0xC3).all())
 
 
+@pytest.mark.parametrize("w", [1, 4, 8])
+def test_block_prefix_path_unaligned(O, w):
+    """Columns past 2^26 elements take the block prefixes (popcount, prefix
+    kernel, writeout).  Input, mask and output off their 16-byte alignment
+    (the unaligned popcount and tile bodies), both bit orders, a ragged
+    length, the staged tile at two shapes and the sparse writeout."""
+    n = (1 << 26) + 4097
+    ioff, moff, ooff = 3, 5, 1
+    data = O.fill_values(n + ioff, NP[w], seed=w + 11)
+    d_data = dev(data)
+    for order in (oracle.MSB_FIRST, oracle.LSB_FIRST):
+        mask_full = dirty_tail(O.mask_bernoulli(n + 8 * moff, 0.4, seed=order + w, bit_order=order)[: (n + 7) // 8 + moff], n + 8 * moff, order)
+        mask = mask_full[moff:]
+        expect, ke = O.compact(data[ioff:], mask, n, order)
+        d_mask_full, d_exp = dev(mask_full), dev(expect)
+        for pipeline, chunk, vpt, block in ((csx.PIPELINE_THREE_PHASE, 4096, 2, 256), (csx.PIPELINE_THREE_PHASE, 1024, 8, 512),
+                                            (csx.PIPELINE_SPARSE, 4096, 8, 256)):
+            print(f"block prefixes unaligned w={w} n={n} order={order} pipeline={pipeline} vpt={vpt} block={block}", flush=True)
+            d_outbuf = torch.full(((n + 16) * w,), 0xC3, dtype=torch.uint8, device="cuda").view(d_data.dtype)
+            d_out = d_outbuf[ooff:]
+            _, k = csx.compact_config(d_data[ioff:], d_mask_full[moff:], csx.Config(pipeline, chunk, vpt, 0, block), n=n, bit_order=order,
+                                      out=d_out)
+            assert k == ke and torch.equal(d_out[:k], d_exp)
+            assert bool((d_out[k:].view(torch.uint8) == 0xC3).all())
+
+
 @pytest.mark.parametrize("vpt", [2, 4, 8])
 @pytest.mark.parametrize("w", [1, 2, 4, 8])
 def test_single_launch_tile_sizes(O, w, vpt):
